@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Benchmark of the CLUE2D batched path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 256 CLUE2D events with hit counts
+log-uniform in [1e3, 1e6] (about 3.4e7 hits), 100 layers, blob + noise model, dc=3, rhoc=10,
+outlierDeltaFactor=2, seeded. A step is one pass of the batched path over the whole batch: one
+hipGraph launch of the recorded kernel sequence, inputs already resident in HBM, timed with HIP
+events on the launch stream. The same events are also run as per-event launches (one launch
+sequence + stream synchronisation per event, as the reference's GPU backend does) and the labels
+of both paths are compared bit for bit.
+
+Prints one JSON line (rank 0, N=1). `--gpus N` with N > 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "heterogeneous-clue_amd"))
+
+import clue_hip  # noqa: E402
+import synth  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_HIT_2D = 36  # in: x, y, weight f32 + layer i32; out: rho, delta f32 + nh, isSeed, clusterIndex i32
+BYTES_PER_CLUSTER_3D = 56
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def ptr(a: clue_hip.DeviceArray, off: int) -> int:
+    return a.ptr + off * a.dtype.itemsize
+
+
+def per_event_run(dev, din, dout, offs, p, reps=1):
+    """Every event as its own launch sequence followed by a stream synchronisation."""
+    lib, ctx = dev.lib, dev.ctx
+    E = offs.size - 1
+    best = None
+    for _ in range(reps):
+        dev.sync()
+        t0 = time.perf_counter()
+        for e in range(E):
+            a, b = int(offs[e]), int(offs[e + 1])
+            if b == a:
+                continue
+            rc = lib.clue2d_run_device(ctx, C.byref(p), b - a, *[ptr(din[k], a) for k in clue_hip.FIELDS_2D_IN],
+                                       *[ptr(dout[k], a) for k in clue_hip.FIELDS_2D_OUT], None)
+            if rc:
+                raise clue_hip.ClueError(rc, dev.error())
+            lib.clue_stream_sync(ctx, None)
+        t = time.perf_counter() - t0
+        best = t if best is None else min(best, t)
+    dev.check()
+    return best
+
+
+def cpu_baseline(packed, offs, p, gpu_out):
+    """The reference's own serial kernels (oracle/_ref) on the host cores, event-parallel."""
+    sys.path.insert(0, str(REPO / "oracle"))
+    import oracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    if oracle.available_reference():
+        R, kind = oracle.Reference(), "reference"
+        t0 = time.perf_counter()
+        out = R.clue2d_batch(packed, offs, p.dc, p.rhoc, p.outlierDeltaFactor, threads=threads)
+        t = time.perf_counter() - t0
+    else:
+        R, kind, threads = oracle.Oracle(), "port", 1
+        t0 = time.perf_counter()
+        out = R.clue2d_batch(packed, offs, p.dc, p.rhoc, p.outlierDeltaFactor)
+        t = time.perf_counter() - t0
+    same = all(np.array_equal(np.ascontiguousarray(out[k]).view(np.int32),
+                              np.ascontiguousarray(gpu_out[k]).view(np.int32)) for k in out)
+    n = int(offs[-1])
+    return {"value": round(n / t / 1e6, 3), "unit": "Mhits/s", "cores": threads, "kind": kind,
+            "sample": f"all {offs.size - 1} events of the workload ({n} hits), one event per thread at a time",
+            "seconds": round(t, 3), "labels_bit_exact_vs_gpu": bool(same)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--events", type=int, default=256)
+    ap.add_argument("--per-event-reps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--no-per-event", action="store_true", help="skip the per-event comparison leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); events are independent, so N GPUs "
+              f"would each run their own batch. Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    counts = synth.c3_hit_counts(args.events)
+    packed, offs = synth.clue2d_batch(counts)
+    N = int(offs[-1])
+    log(f"workload: {args.events} events, {N} hits")
+    dev = clue_hip.Device(0)
+    p = clue_hip.params2d(3.0, 10.0, 2.0)
+    din = dev.upload_soa(packed, clue_hip.FIELDS_2D_IN)
+    dout = dev.alloc_soa(N, clue_hip.FIELDS_2D_OUT)
+    plan = dev.clue2d_plan(din, dout, offs, p, use_graph=True)
+    for _ in range(max(1, args.warmup)):
+        plan.launch()
+    dev.check()
+    lib, ctx = dev.lib, dev.ctx
+    dev.sync()
+    t0 = time.perf_counter()
+    lib.clue_mark(ctx, 0, None)
+    for _ in range(args.steps):
+        plan.launch()
+    lib.clue_mark(ctx, 1, None)
+    dev.sync()
+    wall = time.perf_counter() - t0
+    dev.check()
+    ms = lib.clue_elapsed_ms(ctx, 0, 1) / args.steps
+    log(f"batched graph: {ms:.3f} ms/step (wall {wall * 1e3 / args.steps:.3f})")
+    value = N / (ms * 1e-3) / 1e6
+    batch_out = dev.download_soa(dout)
+
+    result = {
+        "metric": "Mhits/s batched vs per-event launches (bit-exact labels); p50/p99 latency",
+        "value": round(value, 2), "unit": "Mhits/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "clue2d_batched_256_events_loguniform_1e3_1e6", "events": args.events, "hits": N,
+                   "layers": 100, "dc": 3.0, "rhoc": 10.0, "outlierDeltaFactor": 2.0, "launch": "hipGraph"},
+    }
+    algo_bytes = BYTES_PER_HIT_2D * N
+    achieved = algo_bytes / (ms * 1e-3) / 1e9
+    result["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                          "kernel": "batched CLUE2D launch (all kernels of one graph replay)",
+                          "algorithmic_bytes_per_launch": algo_bytes}
+
+    if not args.no_per_event:
+        pout = dev.alloc_soa(N, clue_hip.FIELDS_2D_OUT)
+        t = per_event_run(dev, din, pout, offs, p, reps=args.per_event_reps)
+        per_out = dev.download_soa(pout)
+        same = all(np.array_equal(per_out[k].view(np.int32), batch_out[k].view(np.int32)) for k in per_out)
+        pe = N / t / 1e6
+        result["per_event"] = {"value": round(pe, 2), "unit": "Mhits/s", "ms_total": round(t * 1e3, 3),
+                               "launch": "per event: launch sequence + stream sync",
+                               "labels_bit_exact_vs_batched": bool(same)}
+        result["batched_over_per_event"] = round(value / pe, 2)
+        log(f"per-event: {t * 1e3:.2f} ms total ({pe:.1f} Mhits/s), bit-exact={same}")
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(packed, offs, p, batch_out)
+        log(f"cpu baseline: {result['cpu_baseline']}")
+
+    plan.destroy()
+    dev.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

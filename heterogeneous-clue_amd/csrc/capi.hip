@@ -1,0 +1,735 @@
+// C ABI of the clusterizer: contexts, memory helpers, per-event / batch / plan entry points.
+// See include/clue_hip.h for the contract of each entry point.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/clue_hip.h"
+#include "clue2d.h"
+#include "clue3d.h"
+
+using namespace clue;
+
+namespace {
+
+constexpr int kMarks = 16;
+
+struct Workspace {
+  void* base = nullptr;
+  size_t bytes = 0;
+};
+
+// Sequential carving of one allocation into 256-byte aligned sub-buffers.
+struct Carver {
+  char* p;
+  template <class T>
+  T* take(size_t count) {
+    T* r = reinterpret_cast<T*>(p);
+    p += (count * sizeof(T) + 255) & ~(size_t)255;
+    return r;
+  }
+};
+
+}  // namespace
+
+struct clue_ctx {
+  int device = 0;
+  uint64_t budget = 0;
+  uint64_t inUse = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int* dErr = nullptr;  // device error flag word
+  hipEvent_t marks[kMarks] = {};
+  Workspace scratch2d, scratch3d, hostio;  // run_* workspaces and host-path I/O buffers
+  hipStream_t scratchStream = nullptr;     // stream of the last run_* that used the scratch
+  std::unordered_map<void*, size_t> allocs;
+};
+
+struct clue_plan {
+  clue_ctx* ctx = nullptr;
+  int dim = 2;
+  Workspace ws;
+  Clue2DBuffers b2;
+  Clue2DIO io2{};
+  Clue2DParams p2{};
+  Clue3DBuffers b3;
+  Clue3DIO io3{};
+  Clue3DParams p3{};
+  bool useGraph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int set_err(clue_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_fail(clue_ctx* c, hipError_t e, const char* what) {
+  return set_err(c, CLUE_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr)                                   \
+  do {                                                       \
+    hipError_t _e = (expr);                                  \
+    if (_e != hipSuccess) return hip_fail((ctx), _e, #expr); \
+  } while (0)
+
+hipStream_t pick(clue_ctx* c, void* s) { return s ? reinterpret_cast<hipStream_t>(s) : c->stream; }
+
+int dev_alloc(clue_ctx* c, void** p, size_t bytes) {
+  if (c->budget && c->inUse + bytes > c->budget)
+    return set_err(c, CLUE_ERR_BUDGET, "allocation of " + std::to_string(bytes) + " bytes exceeds the HBM budget (" +
+                                           std::to_string(c->inUse) + " of " + std::to_string(c->budget) +
+                                           " bytes in use)");
+  hipError_t e = hipMalloc(p, bytes ? bytes : 256);
+  if (e != hipSuccess) return hip_fail(c, e, "hipMalloc");
+  c->inUse += bytes;
+  c->allocs[*p] = bytes;
+  return CLUE_OK;
+}
+
+int dev_free(clue_ctx* c, void* p) {
+  if (!p) return CLUE_OK;
+  auto it = c->allocs.find(p);
+  if (it == c->allocs.end()) return set_err(c, CLUE_ERR_ARG, "pointer not allocated by this context");
+  c->inUse -= it->second;
+  c->allocs.erase(it);
+  hipError_t e = hipFree(p);
+  if (e != hipSuccess) return hip_fail(c, e, "hipFree");
+  return CLUE_OK;
+}
+
+int ensure(clue_ctx* c, Workspace& w, size_t bytes) {
+  if (w.bytes >= bytes) return CLUE_OK;
+  if (w.base) {
+    HIP_TRY(c, hipDeviceSynchronize());  // the old workspace may still be read by queued kernels
+    int rc = dev_free(c, w.base);
+    if (rc) return rc;
+    w.base = nullptr;
+    w.bytes = 0;
+  }
+  int rc = dev_alloc(c, &w.base, bytes);
+  if (rc) return rc;
+  w.bytes = bytes;
+  return CLUE_OK;
+}
+
+// The scratch workspace is shared by all run_* calls of a context: calls on another stream wait for
+// the previous user of the scratch first.
+int claim_scratch(clue_ctx* c, hipStream_t s) {
+  if (c->scratchStream && c->scratchStream != s) HIP_TRY(c, hipStreamSynchronize(c->scratchStream));
+  c->scratchStream = s;
+  return CLUE_OK;
+}
+
+bool check_offsets(const int64_t* offs, int n) {
+  if (!offs || n < 0 || offs[0] != 0) return false;
+  for (int e = 0; e < n; ++e)
+    if (offs[e + 1] < offs[e]) return false;
+  return offs[n] < (int64_t)0x7fffffff;
+}
+
+__global__ void k_single_event(int n, int nb, BlockDesc* blocks, int* evOff) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb) {
+    BlockDesc d;
+    d.event = 0;
+    d.begin = b * kSortBlock;
+    d.end = (b + 1) * kSortBlock < n ? (b + 1) * kSortBlock : n;
+    d.histBase = 0;
+    d.nbEvent = nb;
+    d.bLocal = b;
+    blocks[b] = d;
+  }
+  if (b == 0) {
+    evOff[0] = 0;
+    evOff[1] = n;
+  }
+}
+
+template <class B>
+void carve_common(Carver& cv, B& b, int64_t N, int E, int nBlocks, int64_t stride) {
+  const size_t n = (size_t)(N > 0 ? N : 1);
+  b.E = E;
+  b.N = N;
+  b.nBlocks = nBlocks;
+  b.k1 = cv.take<uint32_t>(n);
+  b.i1 = cv.take<int>(n);
+  b.k2 = cv.take<uint32_t>(n);
+  b.i2 = cv.take<int>(n);
+}
+
+void carve2d(void* base, int64_t N, int E, int nBlocks, Clue2DBuffers& b) {
+  Carver cv{reinterpret_cast<char*>(base)};
+  const size_t n = (size_t)(N > 0 ? N : 1);
+  carve_common(cv, b, N, E, nBlocks, clue2d_table_stride());
+  b.xs = cv.take<float>(n);
+  b.ys = cv.take<float>(n);
+  b.ws = cv.take<float>(n);
+  b.rhoS = cv.take<float>(n);
+  b.hist = cv.take<int>((size_t)kRadix * (nBlocks > 0 ? nBlocks : 1));
+  const int64_t maxScan = (int64_t)kRadix * nBlocks > N ? (int64_t)kRadix * nBlocks : N;
+  b.part = cv.take<int>((size_t)scan_partials(maxScan > 0 ? maxScan : 1));
+  b.tableStride = clue2d_table_stride();
+  b.table = cv.take<int>((size_t)b.tableStride * (E > 0 ? E : 1));
+  b.evOff = cv.take<int>((size_t)E + 1);
+  b.blocks = cv.take<BlockDesc>((size_t)(nBlocks > 0 ? nBlocks : 1));
+}
+
+void carve3d(void* base, int64_t N, int E, int nBlocks, Clue3DBuffers& b) {
+  Carver cv{reinterpret_cast<char*>(base)};
+  const size_t n = (size_t)(N > 0 ? N : 1);
+  carve_common(cv, b, N, E, nBlocks, clue3d_table_stride());
+  b.rozS = cv.take<float>(n);
+  b.phiS = cv.take<float>(n);
+  b.enS = cv.take<float>(n);
+  b.rhoS = cv.take<float>(n);
+  b.hist = cv.take<int>((size_t)kRadix * (nBlocks > 0 ? nBlocks : 1));
+  const int64_t maxScan = (int64_t)kRadix * nBlocks > N ? (int64_t)kRadix * nBlocks : N;
+  b.part = cv.take<int>((size_t)scan_partials(maxScan > 0 ? maxScan : 1));
+  b.tableStride = clue3d_table_stride();
+  b.table = cv.take<int>((size_t)b.tableStride * (E > 0 ? E : 1));
+  b.evOff = cv.take<int>((size_t)E + 1);
+  b.blocks = cv.take<BlockDesc>((size_t)(nBlocks > 0 ? nBlocks : 1));
+}
+
+// Upload a host block table and int32 offsets into the workspace (synchronous copies).
+int upload_tables(clue_ctx* c, const int64_t* offsets, int E, const BlockDesc* blocksDev, const int* evOffDev,
+                  std::vector<BlockDesc>& blocks) {
+  blocks = make_blocks(offsets, E);
+  std::vector<int> off32((size_t)E + 1);
+  for (int e = 0; e <= E; ++e) off32[e] = (int)offsets[e];
+  if (!blocks.empty())
+    HIP_TRY(c, hipMemcpy(const_cast<BlockDesc*>(blocksDev), blocks.data(), blocks.size() * sizeof(BlockDesc),
+                         hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(const_cast<int*>(evOffDev), off32.data(), off32.size() * sizeof(int), hipMemcpyHostToDevice));
+  return CLUE_OK;
+}
+
+int nblocks_of(const int64_t* offsets, int E) {
+  int64_t nb = 0;
+  for (int e = 0; e < E; ++e) nb += (offsets[e + 1] - offsets[e] + kSortBlock - 1) / kSortBlock;
+  return (int)nb;
+}
+
+Clue2DParams to2d(const clue2d_params* p) { return Clue2DParams{p->dc, p->rhoc, p->outlierDeltaFactor}; }
+
+Clue3DParams to3d(const clue3d_params* p) {
+  Clue3DParams q;
+  q.densitySiblingLayers = p->densitySiblingLayers;
+  q.densityXYDistanceSqr = p->densityXYDistanceSqr;
+  q.kernelDensityFactor = p->kernelDensityFactor;
+  q.densityOnSameLayer = p->densityOnSameLayer;
+  q.nearestHigherOnSameLayer = p->nearestHigherOnSameLayer;
+  q.criticalXYDistance = p->criticalXYDistance;
+  q.criticalZDistanceLyr = p->criticalZDistanceLyr;
+  q.criticalDensity = p->criticalDensity;
+  q.criticalSelfDensity = p->criticalSelfDensity;
+  q.outlierMultiplier = p->outlierMultiplier;
+  return q;
+}
+
+int single_setup(clue_ctx* c, int n, int nb, const BlockDesc* blocks, const int* evOff, hipStream_t s) {
+  k_single_event<<<(nb + kThreads - 1) / kThreads > 0 ? (nb + kThreads - 1) / kThreads : 1, kThreads, 0, s>>>(
+      n, nb, const_cast<BlockDesc*>(blocks), const_cast<int*>(evOff));
+  HIP_TRY(c, hipGetLastError());
+  return CLUE_OK;
+}
+
+int launch_plan_body(clue_plan* pl, hipStream_t s) {
+  if (pl->dim == 2)
+    clue2d_enqueue(pl->b2, pl->io2, pl->p2, s);
+  else
+    clue3d_enqueue(pl->b3, pl->io3, pl->p3, s);
+  HIP_TRY(pl->ctx, hipGetLastError());
+  return CLUE_OK;
+}
+
+// ---------------------------------------------------------------- arithmetic self-check kernel
+__global__ void k_selftest(int n, const float* v, const float* w, int* xb, int* yb, int* eb, int* pb, float* red,
+                           float* dist, float* quot) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float a = v[i], b = w[i];
+  xb[i] = xbin2d(a);
+  yb[i] = ybin2d(a);
+  eb[i] = eta_bin3d(a);
+  pb[i] = phi_bin3d(a);
+  red[i] = reduce_range(a);
+  dist[i] = sqrtf(a * a + b * b);
+  quot[i] = a / b;
+}
+
+}  // namespace
+
+extern "C" {
+
+clue3d_params clue3d_default_params(void) {
+  clue3d_params p;
+  p.densitySiblingLayers = 3;
+  p.densityXYDistanceSqr = 3.24f;
+  p.kernelDensityFactor = 0.2f;
+  p.densityOnSameLayer = 0;
+  p.nearestHigherOnSameLayer = 0;
+  p.criticalXYDistance = 1.8f;
+  p.criticalZDistanceLyr = 5.f;
+  p.criticalDensity = 0.6f;
+  p.criticalSelfDensity = 0.15f;
+  p.outlierMultiplier = 2.f;
+  return p;
+}
+
+int32_t clue_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+clue_ctx* clue_create(int32_t device, uint64_t hbm_budget_bytes) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return nullptr;
+  if (hipSetDevice(device) != hipSuccess) return nullptr;
+  clue_ctx* c = new (std::nothrow) clue_ctx();
+  if (!c) return nullptr;
+  c->device = device;
+  c->budget = hbm_budget_bytes;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return nullptr;
+  }
+  void* p = nullptr;
+  if (dev_alloc(c, &p, 256) != CLUE_OK) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return nullptr;
+  }
+  c->dErr = reinterpret_cast<int*>(p);
+  fill_i32(c->dErr, 0, 64, c->stream);
+  for (int k = 0; k < kMarks; ++k) (void)hipEventCreate(&c->marks[k]);
+  (void)hipStreamSynchronize(c->stream);
+  return c;
+}
+
+void clue_destroy(clue_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipDeviceSynchronize();
+  for (auto& kv : c->allocs) (void)hipFree(kv.first);
+  c->allocs.clear();
+  for (int k = 0; k < kMarks; ++k)
+    if (c->marks[k]) (void)hipEventDestroy(c->marks[k]);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* clue_last_error(const clue_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+uint64_t clue_bytes_in_use(const clue_ctx* c) { return c ? c->inUse : 0; }
+
+void* clue_stream_create(clue_ctx* c) {
+  if (!c) return nullptr;
+  hipStream_t s = nullptr;
+  (void)hipSetDevice(c->device);
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+  return reinterpret_cast<void*>(s);
+}
+
+int32_t clue_stream_destroy(clue_ctx* c, void* s) {
+  if (!c || !s) return CLUE_ERR_ARG;
+  hipStream_t hs = reinterpret_cast<hipStream_t>(s);
+  if (c->scratchStream == hs) c->scratchStream = nullptr;
+  HIP_TRY(c, hipStreamSynchronize(hs));
+  HIP_TRY(c, hipStreamDestroy(hs));
+  return CLUE_OK;
+}
+
+int32_t clue_stream_sync(clue_ctx* c, void* s) {
+  if (!c) return CLUE_ERR_ARG;
+  HIP_TRY(c, hipStreamSynchronize(pick(c, s)));
+  return CLUE_OK;
+}
+
+void* clue_malloc_device(clue_ctx* c, uint64_t bytes) {
+  if (!c) return nullptr;
+  void* p = nullptr;
+  if (dev_alloc(c, &p, bytes) != CLUE_OK) return nullptr;
+  return p;
+}
+
+int32_t clue_free_device(clue_ctx* c, void* p) {
+  if (!c) return CLUE_ERR_ARG;
+  HIP_TRY(c, hipDeviceSynchronize());
+  return dev_free(c, p);
+}
+
+void* clue_malloc_pinned(clue_ctx* c, uint64_t bytes) {
+  if (!c) return nullptr;
+  void* p = nullptr;
+  hipError_t e = hipHostMalloc(&p, bytes ? bytes : 64, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    hip_fail(c, e, "hipHostMalloc");
+    return nullptr;
+  }
+  return p;
+}
+
+int32_t clue_free_pinned(clue_ctx* c, void* p) {
+  if (!c) return CLUE_ERR_ARG;
+  HIP_TRY(c, hipHostFree(p));
+  return CLUE_OK;
+}
+
+int32_t clue_memcpy_htod(clue_ctx* c, void* dst, const void* src, uint64_t bytes, void* s) {
+  if (!c || (!dst && bytes) || (!src && bytes)) return CLUE_ERR_ARG;
+  if (!bytes) return CLUE_OK;
+  HIP_TRY(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, pick(c, s)));
+  return CLUE_OK;
+}
+
+int32_t clue_memcpy_dtoh(clue_ctx* c, void* dst, const void* src, uint64_t bytes, void* s) {
+  if (!c || (!dst && bytes) || (!src && bytes)) return CLUE_ERR_ARG;
+  if (!bytes) return CLUE_OK;
+  HIP_TRY(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, pick(c, s)));
+  return CLUE_OK;
+}
+
+int32_t clue_check(clue_ctx* c, void* s) {
+  if (!c) return CLUE_ERR_ARG;
+  hipStream_t hs = pick(c, s);
+  HIP_TRY(c, hipStreamSynchronize(hs));
+  int flags = 0;
+  HIP_TRY(c, hipMemcpy(&flags, c->dErr, sizeof(int), hipMemcpyDeviceToHost));
+  if (flags) {
+    fill_i32(c->dErr, 0, 1, hs);
+    HIP_TRY(c, hipStreamSynchronize(hs));
+    if (flags & kErrLayerRange) return set_err(c, CLUE_ERR_LAYER, "input layer outside the tile array");
+    return set_err(c, CLUE_ERR_BIN, "phi outside the range the tile array can bin");
+  }
+  return CLUE_OK;
+}
+
+int32_t clue_mark(clue_ctx* c, int32_t k, void* s) {
+  if (!c || k < 0 || k >= kMarks) return CLUE_ERR_ARG;
+  HIP_TRY(c, hipEventRecord(c->marks[k], pick(c, s)));
+  return CLUE_OK;
+}
+
+float clue_elapsed_ms(clue_ctx* c, int32_t a, int32_t b) {
+  if (!c || a < 0 || b < 0 || a >= kMarks || b >= kMarks) return -1.f;
+  float ms = -1.f;
+  if (hipEventElapsedTime(&ms, c->marks[a], c->marks[b]) != hipSuccess) return -1.f;
+  return ms;
+}
+
+// ---------------------------------------------------------------- CLUE2D
+
+int32_t clue2d_run_device(clue_ctx* c, const clue2d_params* p, int32_t n, const float* x, const float* y,
+                          const int32_t* layer, const float* weight, float* rho, float* delta, int32_t* nh,
+                          int32_t* isSeed, int32_t* clusterIndex, void* stream) {
+  if (!c || !p || n < 0) return set_err(c, CLUE_ERR_ARG, "bad argument");
+  if (n == 0) return CLUE_OK;
+  if (!x || !y || !layer || !weight || !rho || !delta || !nh || !isSeed || !clusterIndex)
+    return set_err(c, CLUE_ERR_ARG, "null buffer");
+  hipStream_t s = pick(c, stream);
+  int rc = claim_scratch(c, s);
+  if (rc) return rc;
+  const int nb = (n + kSortBlock - 1) / kSortBlock;
+  rc = ensure(c, c->scratch2d, clue2d_workspace_bytes(n, 1, nb));
+  if (rc) return rc;
+  Clue2DBuffers b;
+  carve2d(c->scratch2d.base, n, 1, nb, b);
+  b.err = c->dErr;
+  rc = single_setup(c, n, nb, b.blocks, b.evOff, s);
+  if (rc) return rc;
+  Clue2DIO io{x, y, layer, weight, rho, delta, nh, isSeed, clusterIndex};
+  clue2d_enqueue(b, io, to2d(p), s);
+  HIP_TRY(c, hipGetLastError());
+  return CLUE_OK;
+}
+
+int32_t clue2d_run_batch_device(clue_ctx* c, const clue2d_params* p, int32_t E, const int64_t* offsets,
+                                const float* x, const float* y, const int32_t* layer, const float* weight, float* rho,
+                                float* delta, int32_t* nh, int32_t* isSeed, int32_t* clusterIndex, void* stream) {
+  if (!c || !p || !check_offsets(offsets, E)) return set_err(c, CLUE_ERR_ARG, "bad argument or offsets");
+  const int64_t N = offsets[E];
+  if (N == 0) return CLUE_OK;
+  hipStream_t s = pick(c, stream);
+  int rc = claim_scratch(c, s);
+  if (rc) return rc;
+  HIP_TRY(c, hipStreamSynchronize(s));  // the block table below overwrites the shared scratch
+  const int nb = nblocks_of(offsets, E);
+  rc = ensure(c, c->scratch2d, clue2d_workspace_bytes(N, E, nb));
+  if (rc) return rc;
+  Clue2DBuffers b;
+  carve2d(c->scratch2d.base, N, E, nb, b);
+  b.err = c->dErr;
+  std::vector<BlockDesc> blocks;
+  rc = upload_tables(c, offsets, E, b.blocks, b.evOff, blocks);
+  if (rc) return rc;
+  Clue2DIO io{x, y, layer, weight, rho, delta, nh, isSeed, clusterIndex};
+  clue2d_enqueue(b, io, to2d(p), s);
+  HIP_TRY(c, hipGetLastError());
+  return CLUE_OK;
+}
+
+int32_t clue2d_run_host(clue_ctx* c, const clue2d_params* p, int32_t n, const float* x, const float* y,
+                        const int32_t* layer, const float* weight, float* rho, float* delta, int32_t* nh,
+                        int32_t* isSeed, int32_t* clusterIndex) {
+  if (!c || !p || n < 0) return set_err(c, CLUE_ERR_ARG, "bad argument");
+  if (n == 0) return CLUE_OK;
+  const size_t nf = (size_t)n * 4;
+  int rc = claim_scratch(c, c->stream);
+  if (rc) return rc;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  rc = ensure(c, c->hostio, 9 * ((nf + 255) & ~(size_t)255));
+  if (rc) return rc;
+  Carver cv{reinterpret_cast<char*>(c->hostio.base)};
+  float* dx = cv.take<float>(n);
+  float* dy = cv.take<float>(n);
+  int32_t* dl = cv.take<int32_t>(n);
+  float* dw = cv.take<float>(n);
+  float* drho = cv.take<float>(n);
+  float* ddel = cv.take<float>(n);
+  int32_t* dnh = cv.take<int32_t>(n);
+  int32_t* dseed = cv.take<int32_t>(n);
+  int32_t* dci = cv.take<int32_t>(n);
+  hipStream_t s = c->stream;
+  HIP_TRY(c, hipMemcpyAsync(dx, x, nf, hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(dy, y, nf, hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(dl, layer, nf, hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(dw, weight, nf, hipMemcpyHostToDevice, s));
+  rc = clue2d_run_device(c, p, n, dx, dy, dl, dw, drho, ddel, dnh, dseed, dci, s);
+  if (rc) return rc;
+  HIP_TRY(c, hipMemcpyAsync(rho, drho, nf, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(delta, ddel, nf, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(nh, dnh, nf, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(isSeed, dseed, nf, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(clusterIndex, dci, nf, hipMemcpyDeviceToHost, s));
+  return clue_check(c, s);
+}
+
+clue_plan* clue2d_plan_create(clue_ctx* c, const clue2d_params* p, int32_t E, const int64_t* offsets, const float* x,
+                              const float* y, const int32_t* layer, const float* weight, float* rho, float* delta,
+                              int32_t* nh, int32_t* isSeed, int32_t* clusterIndex, int32_t use_graph) {
+  if (!c || !p || !check_offsets(offsets, E)) {
+    set_err(c, CLUE_ERR_ARG, "bad argument or offsets");
+    return nullptr;
+  }
+  clue_plan* pl = new (std::nothrow) clue_plan();
+  if (!pl) return nullptr;
+  pl->ctx = c;
+  pl->dim = 2;
+  const int64_t N = offsets[E];
+  const int nb = nblocks_of(offsets, E);
+  if (ensure(c, pl->ws, clue2d_workspace_bytes(N, E, nb)) != CLUE_OK) {
+    delete pl;
+    return nullptr;
+  }
+  carve2d(pl->ws.base, N, E, nb, pl->b2);
+  pl->b2.err = c->dErr;
+  std::vector<BlockDesc> blocks;
+  if (upload_tables(c, offsets, E, pl->b2.blocks, pl->b2.evOff, blocks) != CLUE_OK) {
+    clue_plan_destroy(pl);
+    return nullptr;
+  }
+  pl->io2 = Clue2DIO{x, y, layer, weight, rho, delta, nh, isSeed, clusterIndex};
+  pl->p2 = to2d(p);
+  pl->useGraph = use_graph != 0;
+  return pl;
+}
+
+// ---------------------------------------------------------------- CLUE3D
+
+int32_t clue3d_run_device(clue_ctx* c, const clue3d_params* p, int32_t n, const float* z, const float* eta,
+                          const float* phi, const float* roz, const float* radius, const float* energy,
+                          const int32_t* layer, const int32_t* isSi, float* rho, float* df, int32_t* dl,
+                          int32_t* nhl, int32_t* nhi, int32_t* isSeed, int32_t* ti, void* stream) {
+  if (!c || !p || n < 0) return set_err(c, CLUE_ERR_ARG, "bad argument");
+  if (n == 0) return CLUE_OK;
+  if (!z || !eta || !phi || !roz || !radius || !energy || !layer || !isSi || !rho || !df || !dl || !nhl || !nhi ||
+      !isSeed || !ti)
+    return set_err(c, CLUE_ERR_ARG, "null buffer");
+  hipStream_t s = pick(c, stream);
+  int rc = claim_scratch(c, s);
+  if (rc) return rc;
+  const int nb = (n + kSortBlock - 1) / kSortBlock;
+  rc = ensure(c, c->scratch3d, clue3d_workspace_bytes(n, 1, nb));
+  if (rc) return rc;
+  Clue3DBuffers b;
+  carve3d(c->scratch3d.base, n, 1, nb, b);
+  b.err = c->dErr;
+  rc = single_setup(c, n, nb, b.blocks, b.evOff, s);
+  if (rc) return rc;
+  Clue3DIO io{z, eta, phi, roz, radius, energy, layer, isSi, rho, df, dl, nhl, nhi, isSeed, ti};
+  clue3d_enqueue(b, io, to3d(p), s);
+  HIP_TRY(c, hipGetLastError());
+  return CLUE_OK;
+}
+
+int32_t clue3d_run_batch_device(clue_ctx* c, const clue3d_params* p, int32_t E, const int64_t* offsets,
+                                const float* z, const float* eta, const float* phi, const float* roz,
+                                const float* radius, const float* energy, const int32_t* layer, const int32_t* isSi,
+                                float* rho, float* df, int32_t* dl, int32_t* nhl, int32_t* nhi, int32_t* isSeed,
+                                int32_t* ti, void* stream) {
+  if (!c || !p || !check_offsets(offsets, E)) return set_err(c, CLUE_ERR_ARG, "bad argument or offsets");
+  const int64_t N = offsets[E];
+  if (N == 0) return CLUE_OK;
+  hipStream_t s = pick(c, stream);
+  int rc = claim_scratch(c, s);
+  if (rc) return rc;
+  HIP_TRY(c, hipStreamSynchronize(s));
+  const int nb = nblocks_of(offsets, E);
+  rc = ensure(c, c->scratch3d, clue3d_workspace_bytes(N, E, nb));
+  if (rc) return rc;
+  Clue3DBuffers b;
+  carve3d(c->scratch3d.base, N, E, nb, b);
+  b.err = c->dErr;
+  std::vector<BlockDesc> blocks;
+  rc = upload_tables(c, offsets, E, b.blocks, b.evOff, blocks);
+  if (rc) return rc;
+  Clue3DIO io{z, eta, phi, roz, radius, energy, layer, isSi, rho, df, dl, nhl, nhi, isSeed, ti};
+  clue3d_enqueue(b, io, to3d(p), s);
+  HIP_TRY(c, hipGetLastError());
+  return CLUE_OK;
+}
+
+int32_t clue3d_run_host(clue_ctx* c, const clue3d_params* p, int32_t n, const float* z, const float* eta,
+                        const float* phi, const float* roz, const float* radius, const float* energy,
+                        const int32_t* layer, const int32_t* isSi, float* rho, float* df, int32_t* dl, int32_t* nhl,
+                        int32_t* nhi, int32_t* isSeed, int32_t* ti) {
+  if (!c || !p || n < 0) return set_err(c, CLUE_ERR_ARG, "bad argument");
+  if (n == 0) return CLUE_OK;
+  const size_t nf = (size_t)n * 4;
+  int rc = claim_scratch(c, c->stream);
+  if (rc) return rc;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  rc = ensure(c, c->hostio, 15 * ((nf + 255) & ~(size_t)255));
+  if (rc) return rc;
+  Carver cv{reinterpret_cast<char*>(c->hostio.base)};
+  const void* in[8] = {z, eta, phi, roz, radius, energy, layer, isSi};
+  void* din[8];
+  for (int k = 0; k < 8; ++k) din[k] = cv.take<float>(n);
+  void* hout[7] = {rho, df, dl, nhl, nhi, isSeed, ti};
+  void* dout[7];
+  for (int k = 0; k < 7; ++k) dout[k] = cv.take<float>(n);
+  hipStream_t s = c->stream;
+  for (int k = 0; k < 8; ++k) HIP_TRY(c, hipMemcpyAsync(din[k], in[k], nf, hipMemcpyHostToDevice, s));
+  rc = clue3d_run_device(c, p, n, (const float*)din[0], (const float*)din[1], (const float*)din[2],
+                         (const float*)din[3], (const float*)din[4], (const float*)din[5], (const int32_t*)din[6],
+                         (const int32_t*)din[7], (float*)dout[0], (float*)dout[1], (int32_t*)dout[2],
+                         (int32_t*)dout[3], (int32_t*)dout[4], (int32_t*)dout[5], (int32_t*)dout[6], s);
+  if (rc) return rc;
+  for (int k = 0; k < 7; ++k) HIP_TRY(c, hipMemcpyAsync(hout[k], dout[k], nf, hipMemcpyDeviceToHost, s));
+  return clue_check(c, s);
+}
+
+clue_plan* clue3d_plan_create(clue_ctx* c, const clue3d_params* p, int32_t E, const int64_t* offsets, const float* z,
+                              const float* eta, const float* phi, const float* roz, const float* radius,
+                              const float* energy, const int32_t* layer, const int32_t* isSi, float* rho, float* df,
+                              int32_t* dl, int32_t* nhl, int32_t* nhi, int32_t* isSeed, int32_t* ti,
+                              int32_t use_graph) {
+  if (!c || !p || !check_offsets(offsets, E)) {
+    set_err(c, CLUE_ERR_ARG, "bad argument or offsets");
+    return nullptr;
+  }
+  clue_plan* pl = new (std::nothrow) clue_plan();
+  if (!pl) return nullptr;
+  pl->ctx = c;
+  pl->dim = 3;
+  const int64_t N = offsets[E];
+  const int nb = nblocks_of(offsets, E);
+  if (ensure(c, pl->ws, clue3d_workspace_bytes(N, E, nb)) != CLUE_OK) {
+    delete pl;
+    return nullptr;
+  }
+  carve3d(pl->ws.base, N, E, nb, pl->b3);
+  pl->b3.err = c->dErr;
+  std::vector<BlockDesc> blocks;
+  if (upload_tables(c, offsets, E, pl->b3.blocks, pl->b3.evOff, blocks) != CLUE_OK) {
+    clue_plan_destroy(pl);
+    return nullptr;
+  }
+  pl->io3 = Clue3DIO{z, eta, phi, roz, radius, energy, layer, isSi, rho, df, dl, nhl, nhi, isSeed, ti};
+  pl->p3 = to3d(p);
+  pl->useGraph = use_graph != 0;
+  return pl;
+}
+
+// ---------------------------------------------------------------- plans
+
+int32_t clue_plan_launch(clue_plan* pl, void* stream) {
+  if (!pl) return CLUE_ERR_ARG;
+  clue_ctx* c = pl->ctx;
+  hipStream_t s = pick(c, stream);
+  if (!pl->useGraph) return launch_plan_body(pl, s);
+  if (!pl->exec) {
+    // Record the launch sequence once on a private stream, then replay it.
+    hipStream_t cap = nullptr;
+    HIP_TRY(c, hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    HIP_TRY(c, hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    int rc = launch_plan_body(pl, cap);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(cap, &g);
+    (void)hipStreamDestroy(cap);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+    pl->graph = g;
+    HIP_TRY(c, hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0));
+  }
+  HIP_TRY(c, hipGraphLaunch(pl->exec, s));
+  return CLUE_OK;
+}
+
+void clue_plan_destroy(clue_plan* pl) {
+  if (!pl) return;
+  clue_ctx* c = pl->ctx;
+  (void)hipDeviceSynchronize();
+  if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+  if (pl->graph) (void)hipGraphDestroy(pl->graph);
+  if (pl->ws.base) dev_free(c, pl->ws.base);
+  delete pl;
+}
+
+uint64_t clue_plan_workspace_bytes(const clue_plan* pl) { return pl ? pl->ws.bytes : 0; }
+
+// ---------------------------------------------------------------- self-check
+
+int32_t clue_selftest_arith(clue_ctx* c, int32_t n, const float* v, const float* w, int32_t* xb, int32_t* yb,
+                            int32_t* eb, int32_t* pb, float* red, float* dist, float* quot) {
+  if (!c || n < 0 || !v || !w) return set_err(c, CLUE_ERR_ARG, "bad argument");
+  if (n == 0) return CLUE_OK;
+  const size_t nf = (size_t)n * 4;
+  Workspace tmp;
+  int rc = ensure(c, tmp, 9 * ((nf + 255) & ~(size_t)255));
+  if (rc) return rc;
+  Carver cv{reinterpret_cast<char*>(tmp.base)};
+  float* dv = cv.take<float>(n);
+  float* dw = cv.take<float>(n);
+  int* o[4];
+  for (int k = 0; k < 4; ++k) o[k] = cv.take<int>(n);
+  float* f[3];
+  for (int k = 0; k < 3; ++k) f[k] = cv.take<float>(n);
+  hipStream_t s = c->stream;
+  HIP_TRY(c, hipMemcpyAsync(dv, v, nf, hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(dw, w, nf, hipMemcpyHostToDevice, s));
+  k_selftest<<<(n + kThreads - 1) / kThreads, kThreads, 0, s>>>(n, dv, dw, o[0], o[1], o[2], o[3], f[0], f[1], f[2]);
+  HIP_TRY(c, hipGetLastError());
+  int32_t* ho[4] = {xb, yb, eb, pb};
+  float* hf[3] = {red, dist, quot};
+  for (int k = 0; k < 4; ++k)
+    if (ho[k]) HIP_TRY(c, hipMemcpyAsync(ho[k], o[k], nf, hipMemcpyDeviceToHost, s));
+  for (int k = 0; k < 3; ++k)
+    if (hf[k]) HIP_TRY(c, hipMemcpyAsync(hf[k], f[k], nf, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  return dev_free(c, tmp.base);
+}
+
+}  // extern "C"

@@ -1,0 +1,58 @@
+// CLUE2D launch sequence (device buffers and parameters).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tiles.h"
+
+namespace clue {
+
+struct Clue2DParams {
+  float dc;                  // critical distance (cm)
+  float rhoc;                // critical density
+  float outlierDeltaFactor;  // dm = outlierDeltaFactor * dc
+};
+
+// Device workspace of one batch plan. Sizes: N points, E events, nBlocks sort blocks.
+struct Clue2DBuffers {
+  int E = 0;
+  int64_t N = 0;
+  int nBlocks = 0;
+  const BlockDesc* blocks = nullptr;  // device
+  const int* evOff = nullptr;         // device, E + 1 (int32 offsets)
+  uint32_t* k1 = nullptr;
+  int* i1 = nullptr;
+  uint32_t* k2 = nullptr;
+  int* i2 = nullptr;
+  float* xs = nullptr;
+  float* ys = nullptr;
+  float* ws = nullptr;
+  float* rhoS = nullptr;
+  int* hist = nullptr;
+  int* part = nullptr;
+  int* table = nullptr;
+  int64_t tableStride = 0;
+  int* err = nullptr;
+};
+
+// Caller-owned device inputs/outputs, packed over the batch (nearestHigher and clusterIndex are
+// event-local indices, as each event's PointsCloudSerial holds them).
+struct Clue2DIO {
+  const float* x;
+  const float* y;
+  const int32_t* layer;
+  const float* weight;
+  float* rho;
+  float* delta;
+  int32_t* nearestHigher;
+  int32_t* isSeed;
+  int32_t* clusterIndex;
+};
+
+constexpr int64_t clue2d_table_stride() { return ((int64_t)k2dKeys + 1 + 63) / 64 * 64; }
+
+void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DParams& p, hipStream_t s);
+size_t clue2d_workspace_bytes(int64_t N, int nEvents, int nBlocks);
+
+}  // namespace clue

@@ -1,0 +1,71 @@
+// CLUE3D (SoA path) launch sequence (device buffers and parameters).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tiles.h"
+
+namespace clue {
+
+// Kernel parameters of KernelCalculateDensitySoA / KernelComputeDistanceToHigherSoA /
+// KernelFindAndAssignClustersSoA (src/serial/plugin-CLUETracksterizer/CLUE3DAlgoKernels.h:26-32,
+// 286-290, 509-515). The reference's compiled-in defaults are listed in clue3d_default_params().
+struct Clue3DParams {
+  int32_t densitySiblingLayers;
+  float densityXYDistanceSqr;
+  float kernelDensityFactor;
+  int32_t densityOnSameLayer;
+  int32_t nearestHigherOnSameLayer;
+  float criticalXYDistance;
+  float criticalZDistanceLyr;
+  float criticalDensity;
+  float criticalSelfDensity;
+  float outlierMultiplier;
+};
+
+struct Clue3DBuffers {
+  int E = 0;
+  int64_t N = 0;
+  int nBlocks = 0;
+  const BlockDesc* blocks = nullptr;
+  const int* evOff = nullptr;
+  uint32_t* k1 = nullptr;
+  int* i1 = nullptr;
+  uint32_t* k2 = nullptr;
+  int* i2 = nullptr;
+  float* rozS = nullptr;
+  float* phiS = nullptr;
+  float* enS = nullptr;
+  float* rhoS = nullptr;
+  int* hist = nullptr;
+  int* part = nullptr;
+  int* table = nullptr;
+  int64_t tableStride = 0;
+  int* err = nullptr;
+};
+
+struct Clue3DIO {
+  const float* z;
+  const float* eta;
+  const float* phi;
+  const float* r_over_absz;
+  const float* radius;
+  const float* energy;
+  const int32_t* layer;
+  const int32_t* isSilicon;
+  float* rho;
+  float* delta_f;         // delta.first
+  int32_t* delta_l;       // delta.second (|layer difference|)
+  int32_t* nh_layer;      // nearestHigher.first
+  int32_t* nh_idx;        // nearestHigher.second (event-local)
+  int32_t* isSeed;
+  int32_t* tracksterIndex;
+};
+
+constexpr int64_t clue3d_table_stride() { return ((int64_t)k3dKeys + 1 + 63) / 64 * 64; }
+
+void clue3d_enqueue(const Clue3DBuffers& b, const Clue3DIO& io, const Clue3DParams& p, hipStream_t s);
+size_t clue3d_workspace_bytes(int64_t N, int nEvents, int nBlocks);
+
+}  // namespace clue

@@ -1,0 +1,158 @@
+// Device-side building blocks shared by the CLUE2D and CLUE3D HIP kernels (gfx950, wave64).
+//
+// Arithmetic contract: every expression that reaches a result is evaluated in IEEE fp32 (fp64
+// where the reference uses double), with the reference's operand order, no FMA contraction
+// (the library is compiled with -ffp-contract=off and this header also pins it with a pragma),
+// correctly rounded sqrt/division, and denormals preserved.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace clue {
+
+// Device-side error bits, OR-ed into the context's flag word and reported after the run.
+constexpr int kErrLayerRange = 1;  // a layer outside the tile array (reference: out-of-bounds access)
+constexpr int kErrBinRange = 2;    // a phi bin outside [0, 126] (phi far outside [-pi, pi])
+
+constexpr int kWave = 64;
+constexpr int kThreads = 256;              // every kernel of this library runs 256-thread blocks
+constexpr int kWaves = kThreads / kWave;
+
+// ---------------------------------------------------------------- float -> int as on x86
+// The reference converts with `int v = float_expr;` on x86-64 (cvttss2si / cvttsd2si): truncation
+// toward zero, and INT_MIN for NaN or values outside the int32 range. The GPU conversion saturates
+// instead, so the out-of-range case is spelled out.
+__device__ __forceinline__ int f2i_x86(float v) {
+  return (v >= -2147483648.0f && v < 2147483648.0f) ? (int)v : (int)0x80000000;
+}
+__device__ __forceinline__ int d2i_x86(double v) {
+  return (v >= -2147483648.0 && v < 2147483648.0) ? (int)v : (int)0x80000000;
+}
+
+// ---------------------------------------------------------------- CLUE2D binning
+// src/serial/DataFormats/LayerTilesConstants.h:7-27 and LayerTilesSerial.h:24-52:
+// 100 layers of 100 x 100 tiles of 5 cm over [-250, 250)^2; rX = 100 / 500.f in float.
+constexpr int k2dLayers = 100;
+constexpr int k2dCols = 100;
+constexpr int k2dRows = 100;
+constexpr int k2dBinsPerLayer = k2dCols * k2dRows;
+constexpr int k2dKeys = k2dLayers * k2dBinsPerLayer;  // key = layer*10^4 + xBin*100 + yBin (x-major)
+
+__device__ __forceinline__ int xbin2d(float x) {
+  const float rX = (float)k2dCols / (250.f - (-250.f));
+  int b = f2i_x86((x - (-250.f)) * rX);
+  b = b < k2dCols - 1 ? b : k2dCols - 1;
+  return b > 0 ? b : 0;
+}
+__device__ __forceinline__ int ybin2d(float y) {
+  const float rY = (float)k2dRows / (250.f - (-250.f));
+  int b = f2i_x86((y - (-250.f)) * rY);
+  b = b < k2dRows - 1 ? b : k2dRows - 1;
+  return b > 0 ? b : 0;
+}
+
+// ---------------------------------------------------------------- CLUE3D binning
+// src/serial/DataFormats/Common.h:9-16, TICLLayerTile.h:15-54, Math/deltaPhi.h:17-24.
+constexpr int k3dLayers = 94;
+constexpr int k3dEta = 34;
+constexpr int k3dPhi = 126;
+constexpr int k3dBinsPerLayer = k3dEta * k3dPhi;
+// Flat key = layer*4284 + etaBin*126 + phiBin, plus one spare row of 126 always-empty bins after
+// the last layer: a window row etaBin == 34 addresses (layer+1, eta 0, phi), as the reference's
+// contiguous tile array does, and reads empty bins past layer 93.
+constexpr int k3dKeys = k3dLayers * k3dBinsPerLayer + k3dPhi;
+constexpr int k3dLastPerSide = k3dLayers / 2;  // 47
+
+__device__ __forceinline__ float reduce_range(float x) {
+  const float o2pi = (float)(1. / (2. * 3.14159265358979323846));
+  if (fabsf(x) <= (float)3.14159265358979323846) return x;
+  const float n = roundf(x * o2pi);
+  return x - n * (float)(2. * 3.14159265358979323846);
+}
+__device__ __forceinline__ int eta_bin3d(float eta) {
+  const float r = (float)k3dEta / (3.2f - 1.5f);
+  int b = f2i_x86((fabsf(eta) - 1.5f) * r);
+  b = b > 0 ? b : 0;
+  return b < k3dEta - 1 ? b : k3dEta - 1;
+}
+__device__ __forceinline__ int phi_bin3d(float phi) {
+  const float normPhi = reduce_range(phi);
+  const float r = (float)(k3dPhi * 0.318309886183790671538 * 0.5f);
+  return d2i_x86(((double)normPhi + 3.14159265358979323846) * (double)r);
+}
+
+// ---------------------------------------------------------------- wave / block primitives
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+// Mask of the lanes of this wave whose `digit` (< 128) equals this lane's digit.
+__device__ __forceinline__ uint64_t match_digit7(int digit) {
+  uint64_t m = ~0ull;
+#pragma unroll
+  for (int b = 0; b < 7; ++b) {
+    const uint64_t bal = __ballot((digit >> b) & 1);
+    m &= ((digit >> b) & 1) ? bal : ~bal;
+  }
+  return m;
+}
+
+__device__ __forceinline__ int wave_inclusive_sum(int v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const int o = __shfl_up(v, off, kWave);
+    if (lane >= off) v += o;
+  }
+  return v;
+}
+
+// Exclusive prefix sum over the 256 threads of a block; `total` receives the block sum.
+// `s_tmp` needs kWaves ints. Contains two barriers: call from uniform control flow.
+__device__ __forceinline__ int block_exclusive_sum(int v, int* s_tmp, int& total) {
+  const int lane = lane_id(), wave = threadIdx.x / kWave;
+  const int inc = wave_inclusive_sum(v);
+  if (lane == kWave - 1) s_tmp[wave] = inc;
+  __syncthreads();
+  int base = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) {
+    const int s = s_tmp[w];
+    base += (w < wave) ? s : 0;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return base + inc - v;
+}
+
+__device__ __forceinline__ int wave_inclusive_min(int v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const int o = __shfl_up(v, off, kWave);
+    if (lane >= off) v = o < v ? o : v;
+  }
+  return v;
+}
+
+// Exclusive prefix min over the block (identity INT_MAX).
+__device__ __forceinline__ int block_exclusive_min(int v, int* s_tmp) {
+  const int lane = lane_id(), wave = threadIdx.x / kWave;
+  const int inc = wave_inclusive_min(v);
+  if (lane == kWave - 1) s_tmp[wave] = inc;
+  __syncthreads();
+  int base = 0x7fffffff;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) {
+    const int s = s_tmp[w];
+    if (w < wave) base = s < base ? s : base;
+  }
+  __syncthreads();
+  int excl = __shfl_up(inc, 1, kWave);
+  if (lane == 0) excl = 0x7fffffff;
+  return excl < base ? excl : base;
+}
+
+}  // namespace clue

@@ -1,0 +1,178 @@
+/*
+ * clue_hip.h — C ABI of the MI355X (gfx950) CLUE / CLUE3D clusterizer.
+ *
+ * Plain C: pointers, sizes and opaque handles only. HIP streams travel as `void*` (a hipStream_t;
+ * NULL selects the context's own stream). Entry points return CLUE_OK (0) or a negative code and
+ * never throw; clue_last_error() gives the message. A C++ caller (the EDProducer plugins in
+ * heterogeneous-clue_amd/plugins) turns a nonzero code into std::runtime_error, the reference's
+ * error convention (src/cuda/CUDACore/cudaCheck.h:20-36, bin/main.cc:199-210).
+ *
+ * Threading: one clue_ctx per EDM stream, as the reference builds one module instance per stream
+ * (src/serial/bin/StreamSchedule.cc:24-38). A context is thread-compatible, not thread-safe.
+ *
+ * Outputs are those of the reference's serial backend, bit for bit:
+ *   CLUE2D: PointsCloudSerial {rho, delta, nearestHigher, isSeed, clusterIndex}
+ *           (src/serial/DataFormats/PointsCloud.h:15-43)
+ *   CLUE3D: ClusterCollectionSerial {rho, delta.first, delta.second, nearestHigher.first,
+ *           nearestHigher.second, isSeed, tracksterIndex} (src/serial/DataFormats/ClusterCollection.h:21-51)
+ * nearestHigher / nearestHigher.second / clusterIndex / tracksterIndex are indices within the event.
+ */
+#ifndef CLUE_HIP_H
+#define CLUE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLUE_OK 0
+#define CLUE_ERR_ARG (-1)      /* bad argument (null pointer, negative size, bad offsets) */
+#define CLUE_ERR_HIP (-2)      /* HIP runtime error */
+#define CLUE_ERR_BUDGET (-3)   /* the allocation would exceed the context's HBM budget */
+#define CLUE_ERR_LAYER (-4)    /* an input layer outside the tile array (2D: [0,100), 3D: [0,94)) */
+#define CLUE_ERR_BIN (-5)      /* CLUE3D: a phi so far outside [-pi, pi] that its bin leaves the tile */
+
+typedef struct clue_ctx clue_ctx;
+typedef struct clue_plan clue_plan;
+
+/* CLUE2D parameters: Parameters {dc, rhoc, outlierDeltaFactor} (src/serial/DataFormats/CLUE_config.h:8-13),
+ * produced by CLUESerialClusterizerESProducer from the CSV `dc,rhoc,odf,produceOutput`
+ * (src/serial/plugin-CLUEClusterizer/CLUESerialClusterizerESProducer.cc:17-34). */
+typedef struct {
+  float dc;
+  float rhoc;
+  float outlierDeltaFactor;
+} clue2d_params;
+
+/* CLUE3D parameters: the kernel arguments of KernelCalculateDensitySoA, KernelComputeDistanceToHigherSoA
+ * and KernelFindAndAssignClustersSoA (src/serial/plugin-CLUETracksterizer/CLUE3DAlgoKernels.h:26-32,
+ * 286-290, 509-515). clue3d_default_params() returns the reference's compiled-in defaults. */
+typedef struct {
+  int32_t densitySiblingLayers;     /* 3 */
+  float densityXYDistanceSqr;       /* 3.24 cm^2 */
+  float kernelDensityFactor;        /* 0.2 */
+  int32_t densityOnSameLayer;       /* 0 */
+  int32_t nearestHigherOnSameLayer; /* 0 */
+  float criticalXYDistance;         /* 1.8 cm */
+  float criticalZDistanceLyr;       /* 5 */
+  float criticalDensity;            /* 0.6 GeV */
+  float criticalSelfDensity;        /* 0.15 */
+  float outlierMultiplier;          /* 2 */
+} clue3d_params;
+
+clue3d_params clue3d_default_params(void);
+
+/* ---------------------------------------------------------------- context, streams, memory */
+
+/* Replaces the per-stream device binding of ScopedContextProduce (src/cuda/CUDACore/ScopedContext.h:106-160,
+ * chooseDevice.cc:5-13) and the caching allocator's budget (getCachingDeviceAllocator.h:14-94).
+ * hbm_budget_bytes = 0: no limit. */
+clue_ctx* clue_create(int32_t device, uint64_t hbm_budget_bytes);
+void clue_destroy(clue_ctx* ctx);
+const char* clue_last_error(const clue_ctx* ctx);
+int32_t clue_device_count(void);
+uint64_t clue_bytes_in_use(const clue_ctx* ctx);
+
+void* clue_stream_create(clue_ctx* ctx);
+int32_t clue_stream_destroy(clue_ctx* ctx, void* stream);
+int32_t clue_stream_sync(clue_ctx* ctx, void* stream);
+
+/* Device / pinned-host memory owned by the caller, counted against the budget (device only). */
+void* clue_malloc_device(clue_ctx* ctx, uint64_t bytes);
+int32_t clue_free_device(clue_ctx* ctx, void* ptr);
+void* clue_malloc_pinned(clue_ctx* ctx, uint64_t bytes);
+int32_t clue_free_pinned(clue_ctx* ctx, void* ptr);
+/* Asynchronous on `stream` (stream-ordered); pinned host memory makes them truly asynchronous. */
+int32_t clue_memcpy_htod(clue_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream);
+int32_t clue_memcpy_dtoh(clue_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream);
+
+/* Synchronise `stream` and report errors the kernels flagged since the last check
+ * (CLUE_ERR_LAYER, CLUE_ERR_BIN). */
+int32_t clue_check(clue_ctx* ctx, void* stream);
+
+/* Elapsed milliseconds between two points of `stream`: record with clue_mark, read with clue_elapsed_ms
+ * after synchronising. Marks are HIP events owned by the context (index 0..15). */
+int32_t clue_mark(clue_ctx* ctx, int32_t index, void* stream);
+float clue_elapsed_ms(clue_ctx* ctx, int32_t from_index, int32_t to_index);
+
+/* ---------------------------------------------------------------- CLUE2D */
+
+/* One event, device-resident SoA, asynchronous on `stream`.
+ * Replaces CLUEAlgoCUDA::makeClusters (src/cuda/plugin-CLUEClusterizer/CLUEAlgoCUDA.cu:59-81) with the
+ * serial backend's results (CLUEAlgoSerial::makeClusters, src/serial/plugin-CLUEClusterizer/CLUEAlgoSerial.cc:18-35). */
+int32_t clue2d_run_device(clue_ctx* ctx, const clue2d_params* p, int32_t n, const float* x, const float* y,
+                          const int32_t* layer, const float* weight, float* rho, float* delta, int32_t* nearestHigher,
+                          int32_t* isSeed, int32_t* clusterIndex, void* stream);
+
+/* One event, host SoA in and out, synchronous: the body of an EDProducer::produce
+ * (src/serial/plugin-CLUEClusterizer/CLUESerialClusterizer.cc:29-36). */
+int32_t clue2d_run_host(clue_ctx* ctx, const clue2d_params* p, int32_t n, const float* x, const float* y,
+                        const int32_t* layer, const float* weight, float* rho, float* delta, int32_t* nearestHigher,
+                        int32_t* isSeed, int32_t* clusterIndex);
+
+/* A packed batch of events (event e = points offsets[e] .. offsets[e+1]-1; host offsets, device SoA),
+ * asynchronous. No reference counterpart: the reference runs one event per produce call. */
+int32_t clue2d_run_batch_device(clue_ctx* ctx, const clue2d_params* p, int32_t n_events, const int64_t* offsets,
+                                const float* x, const float* y, const int32_t* layer, const float* weight, float* rho,
+                                float* delta, int32_t* nearestHigher, int32_t* isSeed, int32_t* clusterIndex,
+                                void* stream);
+
+/* A reusable plan for a fixed batch (sizes and buffers): its own workspace, launched with
+ * clue_plan_launch; use_graph != 0 records the launch sequence once into a hipGraph and replays it. */
+clue_plan* clue2d_plan_create(clue_ctx* ctx, const clue2d_params* p, int32_t n_events, const int64_t* offsets,
+                              const float* x, const float* y, const int32_t* layer, const float* weight, float* rho,
+                              float* delta, int32_t* nearestHigher, int32_t* isSeed, int32_t* clusterIndex,
+                              int32_t use_graph);
+
+/* ---------------------------------------------------------------- CLUE3D */
+
+/* One event, device-resident SoA, asynchronous. Replaces CLUE3DAlgoSerial::makeTrackstersSoA
+ * (src/serial/plugin-CLUETracksterizer/CLUE3DAlgoSerial.cc:42-50) with explicit kernel parameters.
+ * ClusterCollection's x and y (ClusterCollection.h:9-10) are not read by the algorithm and not passed. */
+int32_t clue3d_run_device(clue_ctx* ctx, const clue3d_params* p, int32_t n, const float* z, const float* eta,
+                          const float* phi, const float* r_over_absz, const float* radius, const float* energy,
+                          const int32_t* layer, const int32_t* isSilicon, float* rho, float* delta_first,
+                          int32_t* delta_second, int32_t* nh_layer, int32_t* nh_index, int32_t* isSeed,
+                          int32_t* tracksterIndex, void* stream);
+
+/* Host SoA in/out, synchronous: the body of CLUESerialTracksterizer::produce (CLUESerialTracksterizer.cc:29-41). */
+int32_t clue3d_run_host(clue_ctx* ctx, const clue3d_params* p, int32_t n, const float* z, const float* eta,
+                        const float* phi, const float* r_over_absz, const float* radius, const float* energy,
+                        const int32_t* layer, const int32_t* isSilicon, float* rho, float* delta_first,
+                        int32_t* delta_second, int32_t* nh_layer, int32_t* nh_index, int32_t* isSeed,
+                        int32_t* tracksterIndex);
+
+int32_t clue3d_run_batch_device(clue_ctx* ctx, const clue3d_params* p, int32_t n_events, const int64_t* offsets,
+                                const float* z, const float* eta, const float* phi, const float* r_over_absz,
+                                const float* radius, const float* energy, const int32_t* layer,
+                                const int32_t* isSilicon, float* rho, float* delta_first, int32_t* delta_second,
+                                int32_t* nh_layer, int32_t* nh_index, int32_t* isSeed, int32_t* tracksterIndex,
+                                void* stream);
+
+clue_plan* clue3d_plan_create(clue_ctx* ctx, const clue3d_params* p, int32_t n_events, const int64_t* offsets,
+                              const float* z, const float* eta, const float* phi, const float* r_over_absz,
+                              const float* radius, const float* energy, const int32_t* layer,
+                              const int32_t* isSilicon, float* rho, float* delta_first, int32_t* delta_second,
+                              int32_t* nh_layer, int32_t* nh_index, int32_t* isSeed, int32_t* tracksterIndex,
+                              int32_t use_graph);
+
+/* ---------------------------------------------------------------- plans */
+
+int32_t clue_plan_launch(clue_plan* plan, void* stream);
+void clue_plan_destroy(clue_plan* plan);
+uint64_t clue_plan_workspace_bytes(const clue_plan* plan);
+
+/* ---------------------------------------------------------------- arithmetic self-check
+ * Evaluates the device binning and distance helpers on n host values (synchronous), so that the
+ * device arithmetic can be compared with the CPU reference value by value. Any output may be NULL.
+ * xbin/ybin: LayerTilesSerial::getXBin/getYBin; eta_bin/phi_bin: TICLLayerTileT::etaBin/phiBin;
+ * reduce: reco::reduceRange<float>; dist: sqrtf(a*a + b*b) of CLUEAlgoKernels.h:10-12 with a = v[i],
+ * b = w[i]; quot: a / b. */
+int32_t clue_selftest_arith(clue_ctx* ctx, int32_t n, const float* v, const float* w, int32_t* xbin, int32_t* ybin,
+                            int32_t* eta_bin, int32_t* phi_bin, float* reduce, float* dist, float* quot);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLUE_HIP_H */

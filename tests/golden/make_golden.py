@@ -1,0 +1,172 @@
+"""Write the golden fixtures of this directory from the reference's own serial kernels.
+
+Run in a container that holds the reference tree (the compiled reference library is built from it
+by `make -C oracle ref`). Each fixture is an .npz holding the inputs, the parameters and the
+reference's outputs; nothing of the reference's source is stored.
+
+    python tests/golden/make_golden.py
+
+Fixtures:
+  arith.npz        scalar binning helpers (getXBin/getYBin/etaBin/phiBin/reduceRange) on a float sweep
+  clue2d_<case>.npz  CLUE2D events (seeded synthetic events and edge cases)
+  clue3d_<case>.npz  CLUE3D events (seeded synthetic events and edge cases)
+"""
+from __future__ import annotations
+
+import math
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+REPO = HERE.parents[1]
+sys.path.insert(0, str(REPO / "oracle"))
+sys.path.insert(0, str(REPO / "heterogeneous-clue_amd"))
+
+import oracle  # noqa: E402
+import synth  # noqa: E402
+
+F32 = np.float32
+
+
+def ulp_neighbours(vals, k=3):
+    out = []
+    for v in np.asarray(vals, F32):
+        a = v
+        out.append(a)
+        up = dn = a
+        for _ in range(k):
+            up = np.nextafter(up, F32(np.inf))
+            dn = np.nextafter(dn, F32(-np.inf))
+            out += [up, dn]
+    return np.array(out, F32)
+
+
+def arith_inputs() -> np.ndarray:
+    rng = np.random.Generator(np.random.PCG64(7))
+    edges2d = np.arange(-250.0, 255.0, 5.0)
+    eta_edges = 1.5 + np.arange(0, 36) / 20.0
+    phi_edges = -math.pi + np.arange(0, 127) * (2 * math.pi / 126)
+    special = [0.0, -0.0, math.pi, -math.pi, 2 * math.pi, -2 * math.pi, 3 * math.pi, 1e3, -1e3, 4.9999995,
+               3.0, 3.1, 3.15, 3.2, 1.5, -1.5, 250.0, -250.0, 1e6, -1e6]
+    v = np.concatenate([
+        ulp_neighbours(edges2d, 3), ulp_neighbours(eta_edges, 3), ulp_neighbours(-eta_edges, 2),
+        ulp_neighbours(phi_edges, 3), ulp_neighbours(special, 4),
+        rng.uniform(-260, 260, 60000).astype(F32), rng.uniform(-8, 8, 40000).astype(F32),
+        rng.uniform(-100, 100, 20000).astype(F32),
+    ]).astype(F32)
+    return v
+
+
+# ------------------------------------------------------------------ CLUE2D cases
+
+def ev2d(x, y, layer, w):
+    return {"x": np.asarray(x, F32), "y": np.asarray(y, F32), "layer": np.asarray(layer, np.int32),
+            "weight": np.asarray(w, F32)}
+
+
+def cases_2d():
+    rng = np.random.Generator(np.random.PCG64(11))
+    c = {}
+    P = (3.0, 10.0, 2.0)
+    c["one_hit"] = (ev2d([1.0], [2.0], [5], [1.5]), P)
+    c["two_layers"] = (ev2d([1.0, 1.0], [2.0, 2.0], [5, 6], [20.0, 30.0]), P)
+    n = 60
+    c["duplicates"] = (ev2d(np.full(n, 12.5), np.full(n, -7.25), np.full(n, 3), np.ones(n)), P)
+    g = np.arange(-10, 10, 1.0)
+    gx, gy = np.meshgrid(g, g)
+    c["grid_ties"] = (ev2d(gx.ravel(), gy.ravel(), np.zeros(gx.size), np.ones(gx.size)), (1.5, 3.0, 2.0))
+    m = 4000
+    c["outside"] = (ev2d(rng.uniform(-400, 400, m), rng.uniform(-400, 400, m), rng.integers(0, 100, m),
+                         rng.exponential(1.0, m)), (30.0, 5.0, 2.0))
+    edges = ulp_neighbours(np.arange(-250.0, 255.0, 5.0), 2)
+    ex = rng.choice(edges, 6000)
+    ey = rng.choice(edges, 6000)
+    c["bin_edges"] = (ev2d(ex, ey, rng.integers(0, 4, 6000), rng.exponential(1.0, 6000)), (5.0, 3.0, 2.0))
+    e = synth.clue2d_event(3000, 42)
+    e["layer"] = np.where(e["layer"] < 50, 0, 99).astype(np.int32)
+    c["two_layers_only"] = (e, P)
+    k = 3000
+    c["dense_bin"] = (ev2d(rng.uniform(100.0, 104.99, k), rng.uniform(-50.0, -45.01, k), np.full(k, 7),
+                           rng.exponential(1.0, k)), (0.5, 3.0, 2.0))
+    c["c1_event0"] = (synth.clue2d_event(100_000, 0), P)
+    c["shuffled_20k"] = (synth.clue2d_event(20_000, 5, shuffle=True), P)
+    c["toy_config"] = (synth.clue2d_event(20_000, 6), (20.0, 25.0, 2.0))
+    c["hgcal_config"] = (synth.clue2d_event(20_000, 7), (1.3, 0.11, 2.0))
+    c["large_dm"] = (synth.clue2d_event(20_000, 8), (3.0, 10.0, 4.5))
+    return c
+
+
+# ------------------------------------------------------------------ CLUE3D cases
+
+def ev3d(eta, phi, layer, energy, isSi=None, radius=None):
+    eta = np.asarray(eta, F32)
+    phi = np.asarray(phi, F32)
+    layer = np.asarray(layer, np.int32)
+    n = eta.size
+    z = (320.0 + 1.2 * layer).astype(F32)
+    roz = (1.0 / np.sinh(np.abs(eta.astype(np.float64)))).astype(F32)
+    return {"z": z, "eta": eta, "phi": phi, "r_over_absz": roz,
+            "radius": np.full(n, 1.8, F32) if radius is None else np.asarray(radius, F32),
+            "energy": np.asarray(energy, F32), "layer": layer,
+            "isSilicon": np.ones(n, np.int32) if isSi is None else np.asarray(isSi, np.int32)}
+
+
+def cases_3d():
+    rng = np.random.Generator(np.random.PCG64(13))
+    c = {}
+    c["c4_event0_S2"] = (synth.clue3d_event(0), dict(S=2))
+    c["c4_event1_S3"] = (synth.clue3d_event(1), dict(S=3))
+    n = 1500
+    phi = rng.choice(ulp_neighbours([math.pi, -math.pi, np.float32(math.pi) - 0.02, -np.float32(math.pi) + 0.02], 3),
+                     n) + rng.normal(0, 0.06, n)
+    c["phi_wrap"] = (ev3d(rng.uniform(2.0, 2.3, n), phi, rng.integers(10, 16, n), rng.exponential(2.0, n)),
+                     dict(S=2))
+    c["side_split"] = (ev3d(rng.normal(2.2, 0.06, n), rng.normal(0.5, 0.06, n), rng.integers(43, 52, n),
+                            rng.exponential(2.0, n)), dict(S=3))
+    # |eta| in [3.1, 3.2): the eta window reaches row 34 (CLUE3DAlgoKernels.h:73,329), i.e. bins
+    # of the next layer in the reference's contiguous tile array; layers < 93 keep it in bounds.
+    n4 = 2000
+    lay = rng.integers(0, 92, n4)
+    eta = np.concatenate([rng.uniform(3.1, 3.199, n4 // 2), rng.uniform(1.5, 1.56, n4 // 2)])
+    sgn = rng.choice([-1.0, 1.0], n4)
+    c["eta_row34"] = (ev3d(sgn * eta, rng.uniform(-0.3, 0.3, n4), lay, rng.exponential(2.0, n4)), dict(S=2))
+    n5 = 2500
+    c["scintillator"] = (ev3d(rng.normal(1.8, 0.02, n5), rng.normal(-1.0, 0.02, n5), rng.integers(60, 70, n5),
+                              rng.exponential(2.0, n5), isSi=rng.integers(0, 2, n5),
+                              radius=rng.uniform(0.5, 4.0, n5)), dict(S=2))
+    n6 = 200
+    c["ties3d"] = (ev3d(np.full(n6, 2.3), np.full(n6, 0.7), np.repeat(np.arange(20, 25), n6 // 5), np.ones(n6)),
+                   dict(S=2))
+    g = np.arange(0, 12) * 0.05
+    ge, gp = np.meshgrid(2.0 + g, -0.3 + g)
+    nl = ge.size
+    c["grid3d_ties"] = (ev3d(np.tile(ge.ravel(), 6), np.tile(gp.ravel(), 6), np.repeat(np.arange(30, 36), nl),
+                             np.ones(6 * nl)), dict(S=2))
+    c["single"] = (ev3d([2.0], [0.1], [3], [5.0]), dict(S=3))
+    return c
+
+
+def main():
+    oracle.build(with_reference=True)
+    R = oracle.Reference()
+    v = arith_inputs()
+    b = R.binning(v)
+    np.savez_compressed(HERE / "arith.npz", v=v, **b)
+    print("arith", v.size)
+    for name, (ev, (dc, rhoc, odf)) in cases_2d().items():
+        o = R.clue2d(ev["x"], ev["y"], ev["layer"], ev["weight"], dc, rhoc, odf)
+        np.savez_compressed(HERE / f"clue2d_{name}.npz", **{"in_" + k: v for k, v in ev.items()},
+                            params=np.array([dc, rhoc, odf], F32), **{"out_" + k: v for k, v in o.items()})
+        print("clue2d", name, ev["x"].size, "seeds", int(o["isSeed"].sum()))
+    for name, (ev, pk) in cases_3d().items():
+        p = oracle.clue3d_params(**pk)
+        o = R.clue3d(ev, p)
+        np.savez_compressed(HERE / f"clue3d_{name}.npz", **{"in_" + k: v for k, v in ev.items()},
+                            S=np.int32(pk["S"]), **{"out_" + k: v for k, v in o.items()})
+        print("clue3d", name, ev["layer"].size, "seeds", int(o["isSeed"].sum()))
+
+
+if __name__ == "__main__":
+    main()

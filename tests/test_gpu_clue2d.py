@@ -1,0 +1,118 @@
+"""CLUE2D on the GPU, bit-exact against the reference fixtures and the CPU oracle."""
+import numpy as np
+import pytest
+
+import clue_hip
+import helpers
+import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", helpers.golden_names("clue2d"))
+def test_golden_host_path(dev, name):
+    ins, want, extra = helpers.load_case("clue2d", name)
+    dc, rhoc, odf = (float(v) for v in extra["params"])
+    print(f"case clue2d host {name} n={ins['x'].size}", flush=True)
+    got = dev.clue2d_host(ins, clue_hip.params2d(dc, rhoc, odf))
+    helpers.assert_bit_exact(got, want, helpers.OUT2D, name)
+
+
+@pytest.mark.parametrize("name", helpers.golden_names("clue2d"))
+def test_golden_device_path(dev, name):
+    ins, want, extra = helpers.load_case("clue2d", name)
+    dc, rhoc, odf = (float(v) for v in extra["params"])
+    n = ins["x"].size
+    print(f"case clue2d device {name} n={n}", flush=True)
+    din = dev.upload_soa(ins, clue_hip.FIELDS_2D_IN)
+    dout = dev.alloc_soa(n, clue_hip.FIELDS_2D_OUT)
+    dev.clue2d_device(din, dout, n, clue_hip.params2d(dc, rhoc, odf))
+    dev.check()
+    helpers.assert_bit_exact(dev.download_soa(dout), want, helpers.OUT2D, name)
+    for a in list(din.values()) + list(dout.values()):
+        a.free()
+
+
+def test_one_million_hit_event_vs_oracle(dev, oracle_lib):
+    ev = synth.clue2d_event(1_000_000, 0)
+    print("case clue2d 1e6-hit event", flush=True)
+    got = dev.clue2d_host(ev)
+    want = oracle_lib.clue2d(ev["x"], ev["y"], ev["layer"], ev["weight"])
+    helpers.assert_bit_exact(got, want, helpers.OUT2D, "1e6")
+    helpers.check_invariants_2d(ev, got, 3.0, 10.0, 2.0)
+
+
+def mixed_batch():
+    sizes = [1000, 0, 1, 23_456, 150_000, 3_333, 70_000, 2_048, 2_049, 120]
+    evs = [synth.clue2d_event(n, 500 + i, shuffle=(i % 3 == 0)) for i, n in enumerate(sizes)]
+    return synth.pack(evs)
+
+
+def test_batch_equals_per_event_equals_oracle(dev, oracle_lib):
+    packed, offs = mixed_batch()
+    N = int(offs[-1])
+    print(f"case clue2d mixed batch events={offs.size - 1} n={N}", flush=True)
+    din = dev.upload_soa(packed, clue_hip.FIELDS_2D_IN)
+    dout = dev.alloc_soa(N, clue_hip.FIELDS_2D_OUT)
+    dev.clue2d_batch(din, dout, offs)
+    dev.check()
+    batch = dev.download_soa(dout)
+    want = oracle_lib.clue2d_batch(packed, offs)
+    helpers.assert_bit_exact(batch, want, helpers.OUT2D, "batch vs oracle")
+    # each event alone on the device gives the same bits as in the mix
+    for e in range(offs.size - 1):
+        a, b = int(offs[e]), int(offs[e + 1])
+        if b == a:
+            continue
+        one = dev.clue2d_host({k: packed[k][a:b] for k in clue_hip.FIELDS_2D_IN})
+        helpers.assert_bit_exact(one, {k: v[a:b] for k, v in batch.items()}, helpers.OUT2D, f"event {e} alone")
+
+
+def test_plan_graph_replays_are_identical(dev, oracle_lib):
+    packed, offs = mixed_batch()
+    N = int(offs[-1])
+    print(f"case clue2d graph replay n={N}", flush=True)
+    din = dev.upload_soa(packed, clue_hip.FIELDS_2D_IN)
+    dout = dev.alloc_soa(N, clue_hip.FIELDS_2D_OUT)
+    want = oracle_lib.clue2d_batch(packed, offs)
+    for use_graph in (False, True):
+        plan = dev.clue2d_plan(din, dout, offs, use_graph=use_graph)
+        for rep in range(3):
+            for a in dout.values():
+                a.upload(np.full(N, -7, a.dtype))
+            plan.launch()
+            dev.check()
+            helpers.assert_bit_exact(dev.download_soa(dout), want, helpers.OUT2D, f"graph={use_graph} replay {rep}")
+        plan.destroy()
+
+
+def test_invalid_layer_is_reported(dev):
+    ev = synth.clue2d_event(5000, 9)
+    ev["layer"][17] = 100
+    with pytest.raises(clue_hip.ClueError) as e:
+        dev.clue2d_host(ev)
+    assert e.value.code == -4
+    ev["layer"][17] = 3
+    dev.clue2d_host(ev)  # the context stays usable
+
+
+def test_c3_batch_properties_full_size(dev, oracle_lib):
+    counts = synth.c3_hit_counts(256)
+    packed, offs = synth.clue2d_batch(counts)
+    N = int(offs[-1])
+    print(f"case clue2d 256-event batch n={N}", flush=True)
+    din = dev.upload_soa(packed, clue_hip.FIELDS_2D_IN)
+    dout = dev.alloc_soa(N, clue_hip.FIELDS_2D_OUT)
+    plan = dev.clue2d_plan(din, dout, offs, use_graph=True)
+    plan.launch()
+    dev.check()
+    out = dev.download_soa(dout)
+    for e in range(offs.size - 1):
+        a, b = int(offs[e]), int(offs[e + 1])
+        helpers.check_invariants_2d({k: v[a:b] for k, v in packed.items()}, {k: v[a:b] for k, v in out.items()},
+                                    3.0, 10.0, 2.0)
+    for e in (0, int(np.argmin(counts)), int(np.argmax(counts)), 128):
+        a, b = int(offs[e]), int(offs[e + 1])
+        want = oracle_lib.clue2d(packed["x"][a:b], packed["y"][a:b], packed["layer"][a:b], packed["weight"][a:b])
+        helpers.assert_bit_exact({k: v[a:b] for k, v in out.items()}, want, helpers.OUT2D, f"c3 event {e}")
+    plan.destroy()

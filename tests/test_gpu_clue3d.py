@@ -1,0 +1,80 @@
+"""CLUE3D on the GPU, bit-exact against the reference fixtures and the CPU oracle."""
+import numpy as np
+import pytest
+
+import clue_hip
+import helpers
+import oracle
+import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", helpers.golden_names("clue3d"))
+def test_golden_host_path(dev, name):
+    ins, want, extra = helpers.load_case("clue3d", name)
+    print(f"case clue3d host {name} n={ins['layer'].size}", flush=True)
+    got = dev.clue3d_host(ins, clue_hip.params3d(S=int(extra["S"])))
+    helpers.assert_bit_exact(got, want, helpers.OUT3D, name)
+
+
+@pytest.mark.parametrize("name", helpers.golden_names("clue3d"))
+def test_golden_device_path(dev, name):
+    ins, want, extra = helpers.load_case("clue3d", name)
+    n = ins["layer"].size
+    print(f"case clue3d device {name} n={n}", flush=True)
+    din = dev.upload_soa(ins, clue_hip.FIELDS_3D_IN)
+    dout = dev.alloc_soa(n, clue_hip.FIELDS_3D_OUT)
+    dev.clue3d_device(din, dout, n, clue_hip.params3d(S=int(extra["S"])))
+    dev.check()
+    helpers.assert_bit_exact(dev.download_soa(dout), want, helpers.OUT3D, name)
+    for a in list(din.values()) + list(dout.values()):
+        a.free()
+
+
+def test_batch_equals_per_event_equals_oracle(dev, oracle_lib):
+    evs = [synth.clue3d_event(i) for i in range(6)] + [synth.clue3d_event(7, n_layers=94, mean_per_layer=30)]
+    packed, offs = synth.pack(evs)
+    N = int(offs[-1])
+    print(f"case clue3d batch n={N}", flush=True)
+    din = dev.upload_soa(packed, clue_hip.FIELDS_3D_IN)
+    dout = dev.alloc_soa(N, clue_hip.FIELDS_3D_OUT)
+    for S in (2, 3):
+        dev.clue3d_batch(din, dout, offs, clue_hip.params3d(S=S))
+        dev.check()
+        batch = dev.download_soa(dout)
+        want = oracle_lib.clue3d_batch(packed, offs, oracle.clue3d_params(S=S))
+        helpers.assert_bit_exact(batch, want, helpers.OUT3D, f"batch S={S}")
+        for e in range(offs.size - 1):
+            a, b = int(offs[e]), int(offs[e + 1])
+            one = dev.clue3d_host({k: v[a:b] for k, v in packed.items()}, clue_hip.params3d(S=S))
+            helpers.assert_bit_exact(one, {k: v[a:b] for k, v in batch.items()}, helpers.OUT3D, f"event {e}")
+
+
+def test_plan_graph_c4_properties(dev, oracle_lib):
+    packed, offs = synth.clue3d_batch(512)
+    N = int(offs[-1])
+    print(f"case clue3d 512-event batch n={N}", flush=True)
+    din = dev.upload_soa(packed, clue_hip.FIELDS_3D_IN)
+    dout = dev.alloc_soa(N, clue_hip.FIELDS_3D_OUT)
+    plan = dev.clue3d_plan(din, dout, offs, clue_hip.params3d(S=2), use_graph=True)
+    for rep in range(2):
+        plan.launch()
+        dev.check()
+        out = dev.download_soa(dout)
+        for e in (0, 255, 511):
+            a, b = int(offs[e]), int(offs[e + 1])
+            want = oracle_lib.clue3d({k: v[a:b] for k, v in packed.items()}, oracle.clue3d_params(S=2))
+            helpers.assert_bit_exact({k: v[a:b] for k, v in out.items()}, want, helpers.OUT3D, f"c4 event {e}")
+    for e in range(0, 512, 37):
+        a, b = int(offs[e]), int(offs[e + 1])
+        helpers.check_invariants_3d({k: v[a:b] for k, v in packed.items()}, {k: v[a:b] for k, v in out.items()})
+    plan.destroy()
+
+
+def test_invalid_layer_is_reported(dev):
+    ev = synth.clue3d_event(2)
+    ev["layer"][5] = 94
+    with pytest.raises(clue_hip.ClueError) as e:
+        dev.clue3d_host(ev)
+    assert e.value.code == -4

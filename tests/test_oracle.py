@@ -1,0 +1,114 @@
+"""The CPU oracle against the reference's golden fixtures, closed-form binning checks and invariants.
+
+Golden fixtures come from the reference's own serial kernels (tests/golden/make_golden.py)."""
+import math
+
+import numpy as np
+import pytest
+
+import helpers
+import oracle
+
+
+def test_binning_helpers_match_reference_sweep(oracle_lib):
+    z = np.load(helpers.GOLDEN / "arith.npz", allow_pickle=False)
+    v = z["v"]
+    xb = np.array([oracle_lib.xbin(a) for a in v], np.int32)
+    yb = np.array([oracle_lib.ybin(a) for a in v], np.int32)
+    eb = np.array([oracle_lib.eta_bin(a) for a in v], np.int32)
+    pb = np.array([oracle_lib.phi_bin(a) for a in v], np.int32)
+    red = np.array([oracle_lib.reduce_range(a) for a in v], np.float32)
+    assert np.array_equal(xb, z["xbin"])
+    assert np.array_equal(yb, z["ybin"])
+    assert np.array_equal(eb, z["eta_bin"])
+    assert np.array_equal(pb, z["phi_bin"])
+    assert np.array_equal(red.view(np.int32), z["reduce"].view(np.int32))
+
+
+def test_binning_closed_forms(oracle_lib):
+    # worked example: 4.9999995 + 250 rounds to 255.f, 255 * 0.2f -> bin 51
+    assert oracle_lib.xbin(np.float32(4.9999995)) == 51
+    # every 5 cm edge starts its bin (up to the float rounding of 0.2f), bins are monotone and clamp
+    edges = np.arange(-250.0, 250.0, 5.0, dtype=np.float32)
+    b = np.array([oracle_lib.xbin(e) for e in edges])
+    assert np.all(np.abs(b - np.arange(100)) <= 1)
+    grid = np.linspace(-300, 300, 20001, dtype=np.float32)
+    g = np.array([oracle_lib.xbin(a) for a in grid])
+    assert np.all(np.diff(g) >= 0) and g[0] == 0 and g[-1] == 99
+    # eta bins: 34 bins of 0.05 over [1.5, 3.2), symmetric in eta, clamped
+    assert oracle_lib.eta_bin(1.5) == 0 and oracle_lib.eta_bin(-1.5) == 0
+    assert oracle_lib.eta_bin(1.4) == 0 and oracle_lib.eta_bin(10.0) == 33
+    assert oracle_lib.eta_bin(2.0251) == 10 and oracle_lib.eta_bin(-2.0251) == 10
+    # phi bins: 126 bins over [-pi, pi]; float(pi) lands in the last bin
+    assert oracle_lib.phi_bin(np.float32(math.pi)) == 125
+    assert oracle_lib.phi_bin(np.float32(-math.pi)) == 0
+    # float(126 / 2pi) rounds below the exact ratio, so phi = 0 gives pi * r = 62.99999.. -> bin 62
+    r = np.float32(126.0 / (2.0 * math.pi))
+    assert float(r) < 126.0 / (2.0 * math.pi) and int(math.pi * float(r)) == 62
+    assert oracle_lib.phi_bin(0.0) == 62
+    # reduceRange: identity on [-pi, pi], period 2 pi outside
+    assert oracle_lib.reduce_range(1.0) == np.float32(1.0)
+    r = oracle_lib.reduce_range(np.float32(1.0 + 2 * math.pi))
+    assert abs(float(r) - 1.0) < 1e-6
+
+
+@pytest.mark.parametrize("name", helpers.golden_names("clue2d"))
+def test_oracle_clue2d_golden(oracle_lib, name):
+    ins, want, extra = helpers.load_case("clue2d", name)
+    dc, rhoc, odf = (float(v) for v in extra["params"])
+    got = oracle_lib.clue2d(ins["x"], ins["y"], ins["layer"], ins["weight"], dc, rhoc, odf)
+    helpers.assert_bit_exact(got, want, helpers.OUT2D, name)
+    helpers.check_invariants_2d(ins, got, dc, rhoc, odf)
+
+
+@pytest.mark.parametrize("name", helpers.golden_names("clue3d"))
+def test_oracle_clue3d_golden(oracle_lib, name):
+    ins, want, extra = helpers.load_case("clue3d", name)
+    got = oracle_lib.clue3d(ins, oracle.clue3d_params(S=int(extra["S"])))
+    helpers.assert_bit_exact(got, want, helpers.OUT3D, name)
+    helpers.check_invariants_3d(ins, got)
+
+
+def test_fixture_comparison_catches_a_reordered_sum(oracle_lib):
+    """A planted mistake: summing rho in a different order (double accumulation) must be caught."""
+    ins, want, extra = helpers.load_case("clue2d", "shuffled_20k")
+    dc = float(extra["params"][0])
+    x, y, w, lay = ins["x"], ins["y"], ins["weight"], ins["layer"]
+    # brute-force density accumulated in float64 then rounded: same neighbours, different rounding
+    rho = np.zeros(x.size, np.float64)
+    for L in np.unique(lay):
+        idx = np.nonzero(lay == L)[0]
+        dx = x[idx, None] - x[None, idx]
+        dy = y[idx, None] - y[None, idx]
+        d = np.sqrt((dx * dx).astype(np.float32) + (dy * dy).astype(np.float32))
+        f = np.where(idx[:, None] == idx[None, :], 1.0, 0.5) * w[None, idx]
+        rho[idx] = np.where(d <= np.float32(dc), f, 0.0).sum(axis=1)
+    rho32 = rho.astype(np.float32)
+    # the neighbour sets agree (values within a few ulp) ...
+    assert np.allclose(rho32, want["rho"], rtol=1e-5)
+    # ... but the bits do not: only the serial order reproduces the reference
+    with pytest.raises(AssertionError):
+        helpers.assert_bit_exact({"rho": rho32}, want, ["rho"], "planted")
+    got = oracle_lib.clue2d(x, y, lay, w, dc, float(extra["params"][1]), float(extra["params"][2]))
+    helpers.assert_bit_exact(got, want, ["rho"], "oracle")
+
+
+@pytest.mark.skipif(not oracle.available_reference(), reason="reference kernels not built in this container")
+@pytest.mark.parametrize("seed", [101, 102, 103])
+def test_oracle_matches_compiled_reference_random(oracle_lib, seed):
+    import synth
+    R = oracle.Reference()
+    ev = synth.clue2d_event(30_000, seed, shuffle=(seed % 2 == 0))
+    a = oracle_lib.clue2d(ev["x"], ev["y"], ev["layer"], ev["weight"])
+    b = R.clue2d(ev["x"], ev["y"], ev["layer"], ev["weight"])
+    helpers.assert_bit_exact(a, b, helpers.OUT2D, f"2d seed {seed}")
+    e3 = synth.clue3d_event(seed)
+    for S in (2, 3):
+        p = oracle.clue3d_params(S=S)
+        helpers.assert_bit_exact(oracle_lib.clue3d(e3, p), R.clue3d(e3, p), helpers.OUT3D, f"3d seed {seed} S {S}")
+
+
+def test_oracle_rejects_bad_layer(oracle_lib):
+    with pytest.raises(ValueError):
+        oracle_lib.clue2d(np.zeros(3, np.float32), np.zeros(3, np.float32), np.array([0, 100, 1], np.int32),
+                          np.ones(3, np.float32))
