@@ -171,15 +171,14 @@ void carve2d(void* base, int64_t N, int E, int nBlocks, Clue2DBuffers& b) {
   Carver cv{reinterpret_cast<char*>(base)};
   const size_t n = (size_t)(N > 0 ? N : 1);
   carve_common(cv, b, N, E, nBlocks, clue2d_table_stride());
-  b.xs = cv.take<float>(n);
-  b.ys = cv.take<float>(n);
-  b.ws = cv.take<float>(n);
-  b.rhoS = cv.take<float>(n);
+  b.xyw = cv.take<float4>(n);
+  b.xyrp = cv.take<float4>(n);
   b.hist = cv.take<int>((size_t)kRadix * (nBlocks > 0 ? nBlocks : 1));
   const int64_t maxScan = (int64_t)kRadix * nBlocks > N ? (int64_t)kRadix * nBlocks : N;
   b.part = cv.take<int>((size_t)scan_partials(maxScan > 0 ? maxScan : 1));
   b.tableStride = clue2d_table_stride();
   b.table = cv.take<int>((size_t)b.tableStride * (E > 0 ? E : 1));
+  b.tileStart = cv.take<int>((size_t)(table_tiles(k2dKeys) + 1) * (E > 0 ? E : 1));
   b.evOff = cv.take<int>((size_t)E + 1);
   b.blocks = cv.take<BlockDesc>((size_t)(nBlocks > 0 ? nBlocks : 1));
 }
@@ -197,6 +196,7 @@ void carve3d(void* base, int64_t N, int E, int nBlocks, Clue3DBuffers& b) {
   b.part = cv.take<int>((size_t)scan_partials(maxScan > 0 ? maxScan : 1));
   b.tableStride = clue3d_table_stride();
   b.table = cv.take<int>((size_t)b.tableStride * (E > 0 ? E : 1));
+  b.tileStart = cv.take<int>((size_t)(table_tiles(k3dKeys) + 1) * (E > 0 ? E : 1));
   b.evOff = cv.take<int>((size_t)E + 1);
   b.blocks = cv.take<BlockDesc>((size_t)(nBlocks > 0 ? nBlocks : 1));
 }
@@ -220,7 +220,7 @@ int nblocks_of(const int64_t* offsets, int E) {
   return (int)nb;
 }
 
-Clue2DParams to2d(const clue2d_params* p) { return Clue2DParams{p->dc, p->rhoc, p->outlierDeltaFactor}; }
+Clue2DParams to2d(const clue2d_params* p) { return make_clue2d_params(p->dc, p->rhoc, p->outlierDeltaFactor); }
 
 Clue3DParams to3d(const clue3d_params* p) {
   Clue3DParams q;

@@ -12,7 +12,16 @@ struct Clue2DParams {
   float dc;                  // critical distance (cm)
   float rhoc;                // critical density
   float outlierDeltaFactor;  // dm = outlierDeltaFactor * dc
+  // Squared-distance thresholds: sqrtf(v) <= dc  <=>  v <= dc2 (and the same for dm), exactly,
+  // because a correctly rounded sqrt is monotone; see sqrt_le_threshold().
+  float dc2;
+  float dm2;
 };
+
+// Largest float v >= 0 with sqrtf(v) <= d (sqrtf correctly rounded, hence non-decreasing), so that
+// for every v >= +0 or NaN: (sqrtf(v) <= d) == (v <= sqrt_le_threshold(d)).
+float sqrt_le_threshold(float d);
+Clue2DParams make_clue2d_params(float dc, float rhoc, float outlierDeltaFactor);
 
 // Device workspace of one batch plan. Sizes: N points, E events, nBlocks sort blocks.
 struct Clue2DBuffers {
@@ -25,13 +34,12 @@ struct Clue2DBuffers {
   int* i1 = nullptr;
   uint32_t* k2 = nullptr;
   int* i2 = nullptr;
-  float* xs = nullptr;
-  float* ys = nullptr;
-  float* ws = nullptr;
-  float* rhoS = nullptr;
+  float4* xyw = nullptr;   // tile order: {x, y, weight, 0}
+  float4* xyrp = nullptr;  // tile order: {x, y, rho, point index as int bits}
   int* hist = nullptr;
   int* part = nullptr;
   int* table = nullptr;
+  int* tileStart = nullptr;
   int64_t tableStride = 0;
   int* err = nullptr;
 };

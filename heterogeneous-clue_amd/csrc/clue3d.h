@@ -41,6 +41,7 @@ struct Clue3DBuffers {
   int* hist = nullptr;
   int* part = nullptr;
   int* table = nullptr;
+  int* tileStart = nullptr;
   int64_t tableStride = 0;
   int* err = nullptr;
 };

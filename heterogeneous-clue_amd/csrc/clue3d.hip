@@ -272,7 +272,7 @@ void clue3d_enqueue(const Clue3DBuffers& b, const Clue3DIO& io, const Clue3DPara
       EmitLast3D{b.i2, b.k1, b.i1, io.r_over_absz, io.phi, io.energy, b.rozS, b.phiS, b.enS});
   const uint32_t* keyS = b.k1;
   const int* perm = b.i1;
-  build_table(keyS, b.evOff, b.E, k3dKeys, b.tableStride, b.table, s);
+  build_table(keyS, b.blocks, b.nBlocks, b.evOff, b.E, k3dKeys, b.tableStride, b.table, b.tileStart, s);
   k_density3d<<<b.nBlocks, kThreads, 0, s>>>(b.blocks, perm, b.rozS, b.phiS, b.enS, io.z, io.r_over_absz, io.eta,
                                              io.radius, io.layer, io.isSilicon, b.table, b.tableStride, prm, b.rhoS,
                                              io.rho);
@@ -294,6 +294,7 @@ size_t clue3d_workspace_bytes(int64_t N, int nEvents, int nBlocks) {
   const int64_t maxScan = (int64_t)kRadix * nBlocks > N ? (int64_t)kRadix * nBlocks : N;
   add((size_t)scan_partials(maxScan > 0 ? maxScan : 1) * 4);
   add((size_t)clue3d_table_stride() * (size_t)(nEvents > 0 ? nEvents : 1) * 4);
+  add((size_t)(table_tiles(k3dKeys) + 1) * (size_t)(nEvents > 0 ? nEvents : 1) * 4);  // tileStart
   add((size_t)(nEvents + 1) * 4);
   add((size_t)(nBlocks > 0 ? nBlocks : 1) * sizeof(BlockDesc));
   add(256);

@@ -123,9 +123,12 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const BlockDesc* __r
 }
 
 // Dense bin table: table[e * stride + b] = first sorted position of event e whose key is >= b,
-// for b in [0, nKeys]. One block per (event, kTableTile bins); keys of [lo, hi) are scanned once.
-void build_table(const uint32_t* keySorted, const int* evOffDev, int nEvents, int nKeys, int64_t stride, int* table,
-                 hipStream_t s);
+// for b in [0, nKeys]. A first pass over the sorted keys records where each tile of kTableTile
+// bins starts (tileStart, nEvents * (table_tiles(nKeys) + 1) ints); then one block per
+// (event, tile) fills the tile's bins from the keys of that range.
+inline int table_tiles(int nKeys) { return (nKeys + 1 + kTableTile - 1) / kTableTile; }
+void build_table(const uint32_t* keySorted, const BlockDesc* blocksDev, int nBlocks, const int* evOffDev,
+                 int nEvents, int nKeys, int64_t stride, int* table, int* tileStart, hipStream_t s);
 
 // ---------------------------------------------------------------- seed labels
 // clusterIndex[i] = isSeed[root(i)] ? (#seeds of the event before root) : -1, where root(i) is

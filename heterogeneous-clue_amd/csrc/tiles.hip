@@ -92,33 +92,46 @@ void scan_exclusive(const int* in, int* out, int64_t n, int* part, hipStream_t s
 
 // ---------------------------------------------------------------- dense bin table
 
-__device__ __forceinline__ int lower_bound_u32(const uint32_t* __restrict__ a, int lo, int hi, uint32_t v) {
-  while (lo < hi) {
-    const int mid = lo + ((hi - lo) >> 1);
-    if (a[mid] < v)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
 constexpr int kTablePer = kTableTile / kThreads;  // bins per thread
 
+// tileStart[e][k] = first sorted position of event e whose key is >= k * kTableTile, k in [0, tiles].
+__global__ __launch_bounds__(kThreads) void k_tile_starts(const BlockDesc* __restrict__ blocks,
+                                                          const uint32_t* __restrict__ key,
+                                                          const int* __restrict__ evOff, int tiles,
+                                                          int* __restrict__ tileStart) {
+  const BlockDesc bd = blocks[blockIdx.x];
+  const int s = evOff[bd.event], t = evOff[bd.event + 1];
+  int* TS = tileStart + (int64_t)bd.event * (tiles + 1);
+#pragma unroll 1
+  for (int r = 0; r < kSortItems; ++r) {
+    const int p = bd.begin + r * kThreads + (int)threadIdx.x;
+    if (p >= bd.end) break;
+    const int tl = (int)(key[p] / (uint32_t)kTableTile);
+    const int prev = (p == s) ? -1 : (int)(key[p - 1] / (uint32_t)kTableTile);
+    for (int k = prev + 1; k <= tl; ++k) TS[k] = p;
+    if (p == t - 1)
+      for (int k = tl + 1; k <= tiles; ++k) TS[k] = t;
+  }
+}
+
 __global__ __launch_bounds__(kThreads) void k_table(const uint32_t* __restrict__ key, const int* __restrict__ evOff,
-                                                    int tilesPerEvent, int nKeys, int64_t stride,
-                                                    int* __restrict__ table) {
+                                                    const int* __restrict__ tileStart, int tiles, int nKeys,
+                                                    int64_t stride, int* __restrict__ table) {
   __shared__ int s_first[kTableTile];
-  __shared__ int s_lohi[2];
   __shared__ int s_tmp[kWaves];
-  const int e = blockIdx.x / tilesPerEvent;
-  const int b0 = (blockIdx.x % tilesPerEvent) * kTableTile;
+  const int e = blockIdx.x / tiles;
+  const int tile = blockIdx.x % tiles;
+  const int b0 = tile * kTableTile;
   const int b1 = (b0 + kTableTile) < (nKeys + 1) ? (b0 + kTableTile) : (nKeys + 1);
   const int s = evOff[e], t = evOff[e + 1];
-  if (threadIdx.x < 2) s_lohi[threadIdx.x] = lower_bound_u32(key, s, t, (uint32_t)(threadIdx.x ? b1 : b0));
+  int lo = s, hi = s;
+  if (t > s) {  // tile starts are only recorded for non-empty events
+    const int* TS = tileStart + (int64_t)e * (tiles + 1);
+    lo = TS[tile];
+    hi = TS[tile + 1];
+  }
   for (int j = threadIdx.x; j < kTableTile; j += kThreads) s_first[j] = 0x7fffffff;
   __syncthreads();
-  const int lo = s_lohi[0], hi = s_lohi[1];
   // first position of every non-empty bin of the tile (one writer per bin: the run start)
   for (int p = lo + (int)threadIdx.x; p < hi; p += kThreads) {
     const uint32_t k = key[p];
@@ -145,11 +158,12 @@ __global__ __launch_bounds__(kThreads) void k_table(const uint32_t* __restrict__
   }
 }
 
-void build_table(const uint32_t* keySorted, const int* evOffDev, int nEvents, int nKeys, int64_t stride, int* table,
-                 hipStream_t s) {
+void build_table(const uint32_t* keySorted, const BlockDesc* blocksDev, int nBlocks, const int* evOffDev,
+                 int nEvents, int nKeys, int64_t stride, int* table, int* tileStart, hipStream_t s) {
   if (nEvents <= 0) return;
-  const int tiles = (nKeys + 1 + kTableTile - 1) / kTableTile;
-  k_table<<<nEvents * tiles, kThreads, 0, s>>>(keySorted, evOffDev, tiles, nKeys, stride, table);
+  const int tiles = table_tiles(nKeys);
+  if (nBlocks > 0) k_tile_starts<<<nBlocks, kThreads, 0, s>>>(blocksDev, keySorted, evOffDev, tiles, tileStart);
+  k_table<<<nEvents * tiles, kThreads, 0, s>>>(keySorted, evOffDev, tileStart, tiles, nKeys, stride, table);
 }
 
 // ---------------------------------------------------------------- labels

@@ -90,6 +90,21 @@ def cases_2d():
     k = 3000
     c["dense_bin"] = (ev2d(rng.uniform(100.0, 104.99, k), rng.uniform(-50.0, -45.01, k), np.full(k, 7),
                            rng.exponential(1.0, k)), (0.5, 3.0, 2.0))
+    # per layer: a point at the origin and 40 points on circles of radius dc and dm = 2 dc (+- 2e-6);
+    # near the origin the float spacing is ~2e-7, so the squared distances land within a few ulps of
+    # the sqrt threshold on both sides
+    xs, ys, ls = [], [], []
+    for lay in range(100):
+        xs.append(0.0)
+        ys.append(0.0)
+        ls.append(lay)
+        for k in range(40):
+            r = (3.0 if k % 2 == 0 else 6.0) + rng.uniform(-2e-6, 2e-6)
+            th = rng.uniform(0, 2 * math.pi)
+            xs.append(r * math.cos(th))
+            ys.append(r * math.sin(th))
+            ls.append(lay)
+    c["dc_boundary"] = (ev2d(xs, ys, ls, rng.exponential(1.0, len(xs))), P)
     c["c1_event0"] = (synth.clue2d_event(100_000, 0), P)
     c["shuffled_20k"] = (synth.clue2d_event(20_000, 5, shuffle=True), P)
     c["toy_config"] = (synth.clue2d_event(20_000, 6), (20.0, 25.0, 2.0))
