@@ -41,17 +41,18 @@ constexpr int k2dRows = 100;
 constexpr int k2dBinsPerLayer = k2dCols * k2dRows;
 constexpr int k2dKeys = k2dLayers * k2dBinsPerLayer;  // key = layer*10^4 + xBin*100 + yBin (x-major)
 
+// clamp(f2i_x86(v), 0, hi) in fewer instructions: NaN and v >= 2^31 convert to INT_MIN on x86 and
+// clamp to 0; every other v is clamped to [0, hi] before the (then exact) truncation.
+__device__ __forceinline__ int clamp_bin(float v, int hi) {
+  return v < 2147483648.0f ? (int)fminf(fmaxf(v, 0.f), (float)hi) : 0;
+}
 __device__ __forceinline__ int xbin2d(float x) {
   const float rX = (float)k2dCols / (250.f - (-250.f));
-  int b = f2i_x86((x - (-250.f)) * rX);
-  b = b < k2dCols - 1 ? b : k2dCols - 1;
-  return b > 0 ? b : 0;
+  return clamp_bin((x - (-250.f)) * rX, k2dCols - 1);
 }
 __device__ __forceinline__ int ybin2d(float y) {
   const float rY = (float)k2dRows / (250.f - (-250.f));
-  int b = f2i_x86((y - (-250.f)) * rY);
-  b = b < k2dRows - 1 ? b : k2dRows - 1;
-  return b > 0 ? b : 0;
+  return clamp_bin((y - (-250.f)) * rY, k2dRows - 1);
 }
 
 // ---------------------------------------------------------------- CLUE3D binning
@@ -74,9 +75,7 @@ __device__ __forceinline__ float reduce_range(float x) {
 }
 __device__ __forceinline__ int eta_bin3d(float eta) {
   const float r = (float)k3dEta / (3.2f - 1.5f);
-  int b = f2i_x86((fabsf(eta) - 1.5f) * r);
-  b = b > 0 ? b : 0;
-  return b < k3dEta - 1 ? b : k3dEta - 1;
+  return clamp_bin((fabsf(eta) - 1.5f) * r, k3dEta - 1);
 }
 __device__ __forceinline__ int phi_bin3d(float phi) {
   const float normPhi = reduce_range(phi);

@@ -114,10 +114,14 @@ __global__ __launch_bounds__(kThreads) void k_tile_starts(const BlockDesc* __res
   }
 }
 
+// LDS index of bin j of a tile, padded by one word per kTablePer so that the per-thread chunks
+// [16t, 16t+16) fall into distinct banks.
+__device__ __forceinline__ int tpad(int j) { return j + j / kTablePer; }
+
 __global__ __launch_bounds__(kThreads) void k_table(const uint32_t* __restrict__ key, const int* __restrict__ evOff,
                                                     const int* __restrict__ tileStart, int tiles, int nKeys,
                                                     int64_t stride, int* __restrict__ table) {
-  __shared__ int s_first[kTableTile];
+  __shared__ int s_first[kTableTile + kTableTile / kTablePer];
   __shared__ int s_tmp[kWaves];
   const int e = blockIdx.x / tiles;
   const int tile = blockIdx.x % tiles;
@@ -130,32 +134,34 @@ __global__ __launch_bounds__(kThreads) void k_table(const uint32_t* __restrict__
     lo = TS[tile];
     hi = TS[tile + 1];
   }
-  for (int j = threadIdx.x; j < kTableTile; j += kThreads) s_first[j] = 0x7fffffff;
+  for (int j = threadIdx.x; j < kTableTile; j += kThreads) s_first[tpad(j)] = 0x7fffffff;
   __syncthreads();
   // first position of every non-empty bin of the tile (one writer per bin: the run start)
   for (int p = lo + (int)threadIdx.x; p < hi; p += kThreads) {
     const uint32_t k = key[p];
-    if (p == lo || key[p - 1] != k) s_first[k - (uint32_t)b0] = p;
+    if (p == lo || key[p - 1] != k) s_first[tpad((int)(k - (uint32_t)b0))] = p;
   }
   __syncthreads();
   // table[b] = min(first of bins >= b, hi): a suffix minimum. Thread t owns bins
-  // [b0 + (kThreads-1-t)*kTablePer, +kTablePer) so thread order runs from high bins to low bins.
+  // [(kThreads-1-t)*kTablePer, +kTablePer) of the tile, so thread order runs from high bins to low.
   const int j0 = (kThreads - 1 - (int)threadIdx.x) * kTablePer;
   int m = 0x7fffffff;
 #pragma unroll
   for (int k = 0; k < kTablePer; ++k) {
-    const int v = s_first[j0 + k];
+    const int v = s_first[tpad(j0 + k)];
     m = v < m ? v : m;
   }
   int run = block_exclusive_min(m, s_tmp);
   run = run < hi ? run : hi;
-  int* out = table + (int64_t)e * stride + b0;
 #pragma unroll
   for (int k = kTablePer - 1; k >= 0; --k) {
-    const int v = s_first[j0 + k];
+    const int v = s_first[tpad(j0 + k)];
     run = v < run ? v : run;
-    if (b0 + j0 + k < b1) out[j0 + k] = run;
+    s_first[tpad(j0 + k)] = run;
   }
+  __syncthreads();
+  int* out = table + (int64_t)e * stride + b0;
+  for (int j = threadIdx.x; j < b1 - b0; j += kThreads) out[j] = s_first[tpad(j)];
 }
 
 void build_table(const uint32_t* keySorted, const BlockDesc* blocksDev, int nBlocks, const int* evOffDev,
