@@ -1,0 +1,322 @@
+// clue_hip_main: runs the CLUE2D / CLUE3D HIP modules over the events of an input file, with the
+// command line and final report of the reference's driver (src/serial/bin/main.cc:43-240).
+//
+// Each EDM stream is a std::thread owning its own module instances (as StreamSchedule builds its
+// own workers, src/serial/bin/StreamSchedule.cc:24-38); streams pull events from a shared source
+// until maxEvents are done (cycling over the file's events like Source2D/3D::produce,
+// src/serial/bin/Source.cc:152-227). Modules are found by name in plugin libraries loaded with
+// dlopen (src/serial/bin/PluginManager.cc, SharedLibrary.cc).
+#include <dlfcn.h>
+#include <time.h>
+
+#include <atomic>
+#include <chrono>
+#include <cstdint>
+#include <cstdlib>
+#include <exception>
+#include <filesystem>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "DataFormats/CLUE_config.h"
+#include "DataFormats/ClusterCollection.h"
+#include "DataFormats/PointsCloud.h"
+#include "Framework/ESPluginFactory.h"
+#include "Framework/Event.h"
+#include "Framework/EventSetup.h"
+#include "Framework/PluginFactory.h"
+#include "Framework/ProductRegistry.h"
+#include "Framework/Worker.h"
+
+namespace {
+
+  // ------------------------------------------------------------ input (Source.cc:8-58, 88-144)
+  // A file is a sequence of events: uint32 n, then n records of float32 fields
+  // (2D: x, y, layer, weight; 3D: x, y, z, eta, phi, r_over_absz, radius, layer, energy, isSilicon).
+  template <int NF>
+  bool read_event(std::ifstream& in, std::vector<float>& rec) {
+    uint32_t n = 0;
+    in.read(reinterpret_cast<char*>(&n), sizeof(n));
+    if (in.gcount() == 0) return false;
+    if (in.gcount() != sizeof(n)) throw std::runtime_error("truncated event header");
+    rec.resize(static_cast<size_t>(n) * NF);
+    in.read(reinterpret_cast<char*>(rec.data()), static_cast<std::streamsize>(rec.size() * sizeof(float)));
+    if (static_cast<size_t>(in.gcount()) != rec.size() * sizeof(float)) throw std::runtime_error("truncated event records");
+    return true;
+  }
+
+  std::vector<PointsCloud> read2D(std::filesystem::path const& f) {
+    std::ifstream in(f, std::ios::binary);
+    if (!in) throw std::runtime_error("cannot open " + f.string());
+    std::vector<PointsCloud> evs;
+    std::vector<float> rec;
+    while (read_event<4>(in, rec)) {
+      PointsCloud pc;
+      const size_t n = rec.size() / 4;
+      for (size_t i = 0; i < n; ++i) {
+        pc.x.push_back(rec[4 * i]);
+        pc.y.push_back(rec[4 * i + 1]);
+        pc.layer.push_back(static_cast<int>(rec[4 * i + 2]));
+        pc.weight.push_back(rec[4 * i + 3]);
+      }
+      evs.push_back(std::move(pc));
+    }
+    return evs;
+  }
+
+  std::vector<ClusterCollection> read3D(std::filesystem::path const& f) {
+    std::ifstream in(f, std::ios::binary);
+    if (!in) throw std::runtime_error("cannot open " + f.string());
+    std::vector<ClusterCollection> evs;
+    std::vector<float> r;
+    while (read_event<10>(in, r)) {
+      ClusterCollection c;
+      const size_t n = r.size() / 10;
+      for (size_t i = 0; i < n; ++i) {
+        const float* p = &r[10 * i];
+        c.x.push_back(p[0]);
+        c.y.push_back(p[1]);
+        c.z.push_back(p[2]);
+        c.eta.push_back(p[3]);
+        c.phi.push_back(p[4]);
+        c.r_over_absz.push_back(p[5]);
+        c.radius.push_back(p[6]);
+        c.layer.push_back(static_cast<int>(p[7]));
+        c.energy.push_back(p[8]);
+        c.isSilicon.push_back(static_cast<int>(p[9]));
+      }
+      evs.push_back(std::move(c));
+    }
+    return evs;
+  }
+
+  // ------------------------------------------------------------ one EDM stream
+  template <typename Input>
+  class Stream {
+  public:
+    Stream(int id, std::vector<std::string> const& modules) : id_(id) {
+      token_ = reg_.produces<Input>();
+      int index = 1;
+      for (auto const& m : modules) {
+        reg_.beginModuleConstruction(index++);
+        workers_.push_back(edm::PluginFactory::create(m, reg_));
+      }
+    }
+
+    void run(std::vector<Input> const& events, edm::EventSetup const& es, std::atomic<int>& next, int maxEvents,
+             std::atomic<int>& processed) {
+      for (;;) {
+        const int k = next.fetch_add(1);
+        if (k >= maxEvents) return;
+        edm::Event ev(id_, k + 1, reg_);
+        ev.emplace(token_, events[static_cast<size_t>(k) % events.size()]);
+        for (auto& w : workers_) w->doWork(ev, es);
+        processed.fetch_add(1);
+      }
+    }
+
+    void endJob() {
+      for (auto& w : workers_) w->doEndJob();
+    }
+
+  private:
+    int id_;
+    edm::ProductRegistry reg_;
+    edm::EDPutTokenT<Input> token_;
+    std::vector<std::unique_ptr<edm::Worker>> workers_;
+  };
+
+  template <typename Input>
+  int process(std::vector<Input> const& events, std::vector<std::string> const& modules, edm::EventSetup const& es,
+              int numberOfStreams, int maxEvents, double& seconds) {
+    std::vector<std::unique_ptr<Stream<Input>>> streams;
+    for (int s = 0; s < numberOfStreams; ++s) streams.push_back(std::make_unique<Stream<Input>>(s, modules));
+    std::atomic<int> next{0}, processed{0};
+    std::exception_ptr err;
+    std::mutex m;
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::thread> threads;
+    for (auto& s : streams)
+      threads.emplace_back([&, sp = s.get()] {
+        try {
+          sp->run(events, es, next, maxEvents, processed);
+        } catch (...) {
+          std::lock_guard<std::mutex> g(m);
+          if (!err) err = std::current_exception();
+          next.store(maxEvents);  // stop the other streams
+        }
+      });
+    for (auto& t : threads) t.join();
+    seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (err) std::rethrow_exception(err);
+    for (auto& s : streams) s->endJob();
+    return processed.load();
+  }
+
+  double cpu_seconds() {
+    timespec ts;
+    clock_gettime(CLOCK_PROCESS_CPUTIME_ID, &ts);
+    return static_cast<double>(ts.tv_sec) + 1e-9 * static_cast<double>(ts.tv_nsec);
+  }
+
+  void print_help(std::string const& name) {
+    std::cout
+        << name << ": CLUE / CLUE3D on the HIP backend\n\n"
+        << "Usage: " << name
+        << " [--dim 2|3] [--numberOfThreads NT] [--numberOfStreams NS] [--maxEvents ME] [--inputFile PATH]\n"
+        << "       [--configFile PATH] [--outputDir DIR] [--dumpDir DIR] [--plugin LIB] [--empty]\n\n"
+        << "  --dim               2 (CLUE, default) or 3 (CLUE3D)\n"
+        << "  --numberOfThreads   threads (default 1); each runs one EDM stream\n"
+        << "  --numberOfStreams   concurrent events (default: number of threads)\n"
+        << "  --maxEvents         events to process (default: all events of the input file)\n"
+        << "  --inputFile         raw input (2D: uint32 n + n x {x,y,layer,weight} f32 per event;\n"
+        << "                      3D: n x {x,y,z,eta,phi,r_over_absz,radius,layer,energy,isSilicon})\n"
+        << "  --configFile        2D: CSV dc,rhoc,outlierDeltaFactor,produceOutput; 3D: name,value lines\n"
+        << "  --outputDir         where CLUEOutputProducer writes its CSV files (2D, produceOutput = 1)\n"
+        << "  --dumpDir           write bit-exact binary outputs per event (Event_<id>.bin)\n"
+        << "  --plugin            plugin library to load (default: libclue_plugins.so next to this program)\n"
+        << "  --empty             run no algorithm modules (framework overhead only)\n";
+  }
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  std::vector<std::string> args(argv, argv + argc);
+  int dim = 2, numberOfThreads = 1, numberOfStreams = 0, maxEvents = -1;
+  std::filesystem::path inputFile, configFile, outputDir, dumpDir;
+  std::vector<std::string> plugins;
+  bool empty = false;
+  try {
+    for (size_t i = 1; i < args.size(); ++i) {
+      auto next = [&]() -> std::string const& {
+        if (i + 1 >= args.size()) throw std::invalid_argument("missing value after " + args[i]);
+        return args[++i];
+      };
+      if (args[i] == "-h" || args[i] == "--help") {
+        print_help(args[0]);
+        return EXIT_SUCCESS;
+      } else if (args[i] == "--dim") {
+        dim = std::stoi(next());
+      } else if (args[i] == "--numberOfThreads") {
+        numberOfThreads = std::stoi(next());
+      } else if (args[i] == "--numberOfStreams") {
+        numberOfStreams = std::stoi(next());
+      } else if (args[i] == "--maxEvents") {
+        maxEvents = std::stoi(next());
+      } else if (args[i] == "--inputFile") {
+        inputFile = next();
+      } else if (args[i] == "--configFile") {
+        configFile = next();
+      } else if (args[i] == "--outputDir") {
+        outputDir = next();
+      } else if (args[i] == "--dumpDir") {
+        dumpDir = next();
+      } else if (args[i] == "--plugin") {
+        plugins.push_back(next());
+      } else if (args[i] == "--empty") {
+        empty = true;
+      } else {
+        std::cout << "Invalid parameter " << args[i] << std::endl << std::endl;
+        print_help(args[0]);
+        return EXIT_FAILURE;
+      }
+    }
+  } catch (std::exception const& e) {
+    std::cout << "Invalid command line: " << e.what() << std::endl;
+    return EXIT_FAILURE;
+  }
+  if (dim != 2 && dim != 3) {
+    std::cout << "The dimensionality of the algorithm is either 2 or 3!" << std::endl;
+    return EXIT_FAILURE;
+  }
+  if (numberOfThreads <= 0) numberOfThreads = static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
+  if (numberOfStreams <= 0) numberOfStreams = numberOfThreads;
+  if (inputFile.empty() || !std::filesystem::exists(inputFile)) {
+    std::cout << "Input file '" << inputFile.string() << "' does not exist" << std::endl;
+    return EXIT_FAILURE;
+  }
+  if (dim == 2 && (configFile.empty() || !std::filesystem::exists(configFile))) {
+    std::cout << "Config file '" << configFile.string() << "' does not exist" << std::endl;
+    return EXIT_FAILURE;
+  }
+  if (plugins.empty())
+    plugins.push_back((std::filesystem::canonical("/proc/self/exe").parent_path() / "libclue_plugins.so").string());
+  for (auto const& p : plugins)
+    if (!dlopen(p.c_str(), RTLD_NOW | RTLD_GLOBAL)) {
+      std::cout << "Cannot load plugin library " << p << ": " << dlerror() << std::endl;
+      return EXIT_FAILURE;
+    }
+
+  try {
+    edm::EventSetup es;
+    std::vector<std::string> modules;
+    double seconds = 0;
+    int processed = 0;
+    const double cpu0 = cpu_seconds();
+    if (dim == 2) {
+      edm::ESPluginFactory::create("CLUEHIPClusterizerESProducer", configFile)->produce(es);
+      Parameters const& par = es.get<Parameters>();
+      std::cerr << "Running CLUE 2D algorithm with the following parameters: \n"
+                << "dc = " << par.dc << '\n'
+                << "rhoc = " << par.rhoc << '\n'
+                << "outlierDeltaFactor = " << par.outlierDeltaFactor << std::endl;
+      if (!empty) {
+        modules.push_back("CLUEHIPClusterizer");
+        if (par.produceOutput) {
+          edm::ESPluginFactory::create("CLUEOutputESProducer", outputDir.empty() ? "output" : outputDir)->produce(es);
+          modules.push_back("CLUEOutputProducer");
+        }
+        if (!dumpDir.empty()) {
+          edm::ESPluginFactory::create("CLUEDumpESProducer", dumpDir)->produce(es);
+          modules.push_back("CLUEBinaryDump");
+        }
+      }
+      auto events = read2D(inputFile);
+      if (events.empty()) throw std::runtime_error("input file holds no events");
+      if (maxEvents < 0) maxEvents = static_cast<int>(events.size());
+      std::cout << "Processing " << maxEvents << " events, of which " << numberOfStreams << " concurrently, with "
+                << numberOfThreads << " threads." << std::endl;
+      processed = process(events, modules, es, numberOfStreams, maxEvents, seconds);
+    } else {
+      if (!configFile.empty()) {
+        edm::ESPluginFactory::create("CLUEHIPTracksterizerESProducer", configFile)->produce(es);
+        std::cerr << "Running CLUE 3D algorithm with the parameters of " << configFile.string() << "\n";
+      } else {
+        std::cerr << "Running CLUE 3D algorithm with default parameters\n";
+      }
+      if (!empty) {
+        modules.push_back("CLUEHIPTracksterizer");
+        if (!dumpDir.empty()) {
+          edm::ESPluginFactory::create("CLUEDumpESProducer", dumpDir)->produce(es);
+          modules.push_back("CLUE3DBinaryDump");
+        }
+      }
+      auto events = read3D(inputFile);
+      if (events.empty()) throw std::runtime_error("input file holds no events");
+      if (maxEvents < 0) maxEvents = static_cast<int>(events.size());
+      std::cout << "Processing " << maxEvents << " events, of which " << numberOfStreams << " concurrently, with "
+                << numberOfThreads << " threads." << std::endl;
+      processed = process(events, modules, es, numberOfStreams, maxEvents, seconds);
+    }
+    const double cpu = cpu_seconds() - cpu0;
+    std::cout << "Processed " << processed << " events in " << std::scientific << seconds << " seconds, throughput "
+              << std::defaultfloat << (processed / seconds) << " events/s, CPU usage per thread: " << std::fixed
+              << std::setprecision(1) << (cpu / seconds / numberOfThreads * 100) << "%" << std::endl;
+  } catch (std::runtime_error const& e) {
+    std::cout << "\n----------\nCaught std::runtime_error" << std::endl;
+    std::cout << e.what() << std::endl;
+    return EXIT_FAILURE;
+  } catch (std::exception const& e) {
+    std::cout << "\n----------\nCaught std::exception" << std::endl;
+    std::cout << e.what() << std::endl;
+    return EXIT_FAILURE;
+  }
+  return EXIT_SUCCESS;
+}

@@ -1,0 +1,60 @@
+"""The C++ host side above the C ABI on the CPU: framework self-test and driver error handling."""
+import subprocess
+from pathlib import Path
+
+import pytest
+
+import synth
+
+PKG = Path(__file__).resolve().parents[1] / "heterogeneous-clue_amd"
+MAIN = PKG / "clue_hip_main"
+
+
+def run(args, **kw):
+    return subprocess.run([str(MAIN)] + args, capture_output=True, text=True, timeout=120, **kw)
+
+
+def test_framework_selftest():
+    r = subprocess.run([str(PKG / "clue_fw_selftest")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "framework self-test passed" in r.stdout
+    assert r.stdout.count("ok ") == 10
+
+
+def test_driver_help_and_bad_arguments():
+    r = run(["--help"])
+    assert r.returncode == 0 and "--numberOfStreams" in r.stdout
+    r = run(["--bogus"])
+    assert r.returncode != 0 and "Invalid parameter --bogus" in r.stdout
+    r = run(["--dim", "4"])
+    assert r.returncode != 0 and "either 2 or 3" in r.stdout
+    r = run(["--inputFile", "/nonexistent.bin"])
+    assert r.returncode != 0 and "does not exist" in r.stdout
+
+
+def test_driver_truncated_input_and_missing_device(tmp_path):
+    f = tmp_path / "raw2D.bin"
+    synth.write_raw(f, [synth.clue2d_event(500, 1)], dim=2)
+    cfg = tmp_path / "cfg.csv"
+    cfg.write_text("3,10,2,0\n")
+    r = run(["--dim", "2", "--inputFile", str(f)])
+    assert r.returncode != 0 and "Config file" in r.stdout
+    with open(f, "ab") as h:
+        h.write(b"\x10\x00\x00\x00\x01\x02")
+    r = run(["--dim", "2", "--inputFile", str(f), "--configFile", str(cfg)])
+    assert r.returncode != 0 and "truncated" in r.stdout
+
+
+@pytest.mark.skipif(subprocess.run([str(MAIN), "--help"], capture_output=True).returncode != 0,
+                    reason="driver not built")
+def test_driver_without_gpu_fails_cleanly(tmp_path):
+    import clue_hip
+    if clue_hip.load_library().clue_device_count() > 0:
+        pytest.skip("a GPU is present; the GPU tests cover the successful run")
+    f = tmp_path / "raw2D.bin"
+    synth.write_raw(f, [synth.clue2d_event(500, 1)], dim=2)
+    cfg = tmp_path / "cfg.csv"
+    cfg.write_text("3,10,2,0\n")
+    r = run(["--dim", "2", "--inputFile", str(f), "--configFile", str(cfg)])
+    assert r.returncode != 0
+    assert "Caught std::runtime_error" in r.stdout and "no HIP device" in r.stdout

@@ -1,0 +1,83 @@
+"""The driver running the HIP modules through the plugin interface, outputs compared bit for bit."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import helpers
+import oracle
+import synth
+
+pytestmark = pytest.mark.gpu
+PKG = Path(__file__).resolve().parents[1] / "heterogeneous-clue_amd"
+
+
+def read_dump(path, dim):
+    raw = np.fromfile(path, dtype=np.uint8)
+    n = int(raw[:4].view(np.uint32)[0])
+    cols = helpers.OUT2D if dim == 2 else helpers.OUT3D
+    out, pos = {}, 4
+    for k in cols:
+        dt = np.float32 if k in ("rho", "delta", "delta_f") else np.int32
+        out[k] = raw[pos:pos + 4 * n].view(dt)
+        pos += 4 * n
+    assert pos == raw.size
+    return out
+
+
+def test_driver_clue2d_two_streams(tmp_path, oracle_lib):
+    evs = [synth.clue2d_event(n, 70 + i, shuffle=(i == 1)) for i, n in enumerate((3000, 25000, 1200, 60000))]
+    f = tmp_path / "raw2D.bin"
+    synth.write_raw(f, evs, dim=2)
+    cfg = tmp_path / "cfg.csv"
+    cfg.write_text("3,10,2,1\n")
+    print("case driver clue2d", flush=True)
+    r = subprocess.run([str(PKG / "clue_hip_main"), "--dim", "2", "--inputFile", str(f), "--configFile", str(cfg),
+                        "--numberOfThreads", "2", "--maxEvents", "6", "--outputDir", str(tmp_path / "out"),
+                        "--dumpDir", str(tmp_path / "dump")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Processed 6 events in" in r.stdout and "events/s" in r.stdout
+    for eid in range(1, 7):
+        ev = evs[(eid - 1) % len(evs)]
+        got = read_dump(tmp_path / "dump" / f"Event_{eid}.bin", 2)
+        want = oracle_lib.clue2d(ev["x"], ev["y"], ev["layer"], ev["weight"], 3.0, 10.0, 2.0)
+        helpers.assert_bit_exact(got, want, helpers.OUT2D, f"driver event {eid}")
+        csv = tmp_path / "out" / f"Event_{eid}_3.00_10.00_2.00.csv"
+        lines = csv.read_text().splitlines()
+        assert lines[0] == "index,x,y,layer,weight,rho,delta,nh,isSeed,clusterId"
+        assert len(lines) == ev["x"].size + 1
+        last = lines[-1].split(",")
+        assert int(last[-1]) == int(want["clusterIndex"][-1]) and int(last[-2]) == int(want["isSeed"][-1])
+
+
+def test_driver_clue3d_with_parameters(tmp_path, oracle_lib):
+    evs = [synth.clue3d_event(i) for i in range(3)]
+    f = tmp_path / "raw3D.bin"
+    synth.write_raw(f, evs, dim=3)
+    cfg = tmp_path / "clue3d.txt"
+    cfg.write_text("# C4 parameters\ndensitySiblingLayers,2\ncriticalDensity,0.6\n")
+    print("case driver clue3d", flush=True)
+    r = subprocess.run([str(PKG / "clue_hip_main"), "--dim", "3", "--inputFile", str(f), "--configFile", str(cfg),
+                        "--numberOfThreads", "2", "--dumpDir", str(tmp_path / "dump")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Processed 3 events" in r.stdout
+    for eid in range(1, 4):
+        got = read_dump(tmp_path / "dump" / f"Event_{eid}.bin", 3)
+        want = oracle_lib.clue3d(evs[eid - 1], oracle.clue3d_params(S=2))
+        helpers.assert_bit_exact(got, want, helpers.OUT3D, f"driver 3d event {eid}")
+
+
+def test_driver_reports_bad_layer(tmp_path):
+    ev = synth.clue2d_event(2000, 3)
+    ev["layer"][10] = 150
+    f = tmp_path / "raw2D.bin"
+    synth.write_raw(f, [ev], dim=2)
+    cfg = tmp_path / "cfg.csv"
+    cfg.write_text("3,10,2,0\n")
+    print("case driver bad layer", flush=True)
+    r = subprocess.run([str(PKG / "clue_hip_main"), "--dim", "2", "--inputFile", str(f), "--configFile", str(cfg)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "Caught std::runtime_error" in r.stdout and "layer" in r.stdout
