@@ -14,7 +14,8 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libclue_hip.so"
 
 CLUE_OK = 0
-ERRORS = {-1: "CLUE_ERR_ARG", -2: "CLUE_ERR_HIP", -3: "CLUE_ERR_BUDGET", -4: "CLUE_ERR_LAYER", -5: "CLUE_ERR_BIN"}
+ERRORS = {-1: "CLUE_ERR_ARG", -2: "CLUE_ERR_HIP", -3: "CLUE_ERR_BUDGET", -4: "CLUE_ERR_LAYER", -5: "CLUE_ERR_BIN",
+          -6: "CLUE_ERR_TIMEOUT"}
 
 # Every symbol include/clue_hip.h declares.
 EXPORTS = [
@@ -24,7 +25,9 @@ EXPORTS = [
     "clue_check", "clue_mark", "clue_elapsed_ms", "clue2d_run_device", "clue2d_run_host",
     "clue2d_run_batch_device", "clue2d_plan_create", "clue3d_run_device", "clue3d_run_host",
     "clue3d_run_batch_device", "clue3d_plan_create", "clue_plan_launch", "clue_plan_destroy",
-    "clue_plan_workspace_bytes", "clue_selftest_arith",
+    "clue_plan_workspace_bytes", "clue_selftest_arith", "clue_server_create", "clue_server_destroy",
+    "clue_server_submit2d", "clue_server_submit3d", "clue_server_wait", "clue_server_stats",
+    "clue_server_last_error",
 ]
 
 
@@ -92,6 +95,14 @@ def load_library(path: Path = LIB_PATH) -> C.CDLL:
         "clue_plan_destroy": (None, [VP]),
         "clue_plan_workspace_bytes": (C.c_uint64, [VP]),
         "clue_selftest_arith": (I32, [VP, I32, VP, VP] + [VP] * 7),
+        "clue_server_create": (VP, [VP, C.POINTER(Clue2DParams), C.POINTER(Clue3DParams), C.c_int64, C.c_int64,
+                                    I32, I32]),
+        "clue_server_destroy": (None, [VP]),
+        "clue_server_submit2d": (C.c_int64, [VP, I32] + [VP] * 9),
+        "clue_server_submit3d": (C.c_int64, [VP, I32] + [VP] * 15),
+        "clue_server_wait": (I32, [VP, C.c_int64, I32, C.POINTER(C.c_double)]),
+        "clue_server_stats": (I32, [VP] + [C.POINTER(C.c_int64)] * 3),
+        "clue_server_last_error": (C.c_char_p, [VP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -267,6 +278,65 @@ class Device:
                                                *[o[k].ctypes.data for k in ("xbin", "ybin", "eta_bin", "phi_bin",
                                                                             "reduce", "dist", "quot")]))
         return o
+
+
+class Server:
+    """Continuous-batching request loop (clue_server_*). Submitted numpy arrays are kept referenced
+    until the request is waited for."""
+
+    def __init__(self, dev: "Device", p2: Clue2DParams | None = None, p3: Clue3DParams | None = None,
+                 max_points_2d: int = 4_000_000, max_points_3d: int = 1_000_000, max_events: int = 64,
+                 slots: int = 3):
+        self.dev = dev
+        self._p2, self._p3 = p2 or params2d(), p3 or params3d()
+        self.h = dev.lib.clue_server_create(dev.ctx, C.byref(self._p2), C.byref(self._p3), max_points_2d,
+                                            max_points_3d, max_events, slots)
+        if not self.h:
+            raise ClueError(-2, dev.error())
+        self._live = {}
+
+    def _err(self):
+        return self.dev.lib.clue_server_last_error(self.h).decode()
+
+    def submit2d(self, ev: dict) -> int:
+        ins = [np.ascontiguousarray(ev[k], dtype=_dt(k)) for k in FIELDS_2D_IN]
+        n = ins[0].size
+        outs = {k: np.empty(n, _dt(k)) for k in FIELDS_2D_OUT}
+        rid = self.dev.lib.clue_server_submit2d(self.h, n, *[a.ctypes.data for a in ins],
+                                                *[outs[k].ctypes.data for k in FIELDS_2D_OUT])
+        if rid < 0:
+            raise ClueError(int(rid), self._err())
+        self._live[rid] = (ins, outs)
+        return int(rid)
+
+    def submit3d(self, ev: dict) -> int:
+        ins = [np.ascontiguousarray(ev[k], dtype=_dt(k)) for k in FIELDS_3D_IN]
+        n = ins[0].size
+        outs = {k: np.empty(n, _dt(k)) for k in FIELDS_3D_OUT}
+        rid = self.dev.lib.clue_server_submit3d(self.h, n, *[a.ctypes.data for a in ins],
+                                                *[outs[k].ctypes.data for k in FIELDS_3D_OUT])
+        if rid < 0:
+            raise ClueError(int(rid), self._err())
+        self._live[rid] = (ins, outs)
+        return int(rid)
+
+    def wait(self, rid: int, timeout_ms: int = 60_000):
+        lat = C.c_double(0.0)
+        rc = self.dev.lib.clue_server_wait(self.h, rid, timeout_ms, C.byref(lat))
+        if rc != CLUE_OK:
+            raise ClueError(rc, self._err())
+        _, outs = self._live.pop(rid)
+        return outs, lat.value
+
+    def stats(self):
+        b, e, p = C.c_int64(), C.c_int64(), C.c_int64()
+        self.dev.lib.clue_server_stats(self.h, C.byref(b), C.byref(e), C.byref(p))
+        return {"batches": b.value, "events": e.value, "points": p.value}
+
+    def close(self):
+        if self.h:
+            self.dev.lib.clue_server_destroy(self.h)
+            self.h = None
 
 
 class Plan:

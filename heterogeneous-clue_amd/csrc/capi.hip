@@ -13,47 +13,14 @@
 #include "clue2d.h"
 #include "clue3d.h"
 
+#include "context.h"
+
 using namespace clue;
-
-namespace {
-
-constexpr int kMarks = 16;
-
-struct Workspace {
-  void* base = nullptr;
-  size_t bytes = 0;
-};
-
-// Sequential carving of one allocation into 256-byte aligned sub-buffers.
-struct Carver {
-  char* p;
-  template <class T>
-  T* take(size_t count) {
-    T* r = reinterpret_cast<T*>(p);
-    p += (count * sizeof(T) + 255) & ~(size_t)255;
-    return r;
-  }
-};
-
-}  // namespace
-
-struct clue_ctx {
-  int device = 0;
-  uint64_t budget = 0;
-  uint64_t inUse = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  int* dErr = nullptr;  // device error flag word
-  hipEvent_t marks[kMarks] = {};
-  Workspace scratch2d, scratch3d, hostio;  // run_* workspaces and host-path I/O buffers
-  hipStream_t scratchStream = nullptr;     // stream of the last run_* that used the scratch
-  std::unordered_map<void*, size_t> allocs;
-};
 
 struct clue_plan {
   clue_ctx* ctx = nullptr;
   int dim = 2;
-  Workspace ws;
+  clue::detail::Workspace ws;
   Clue2DBuffers b2;
   Clue2DIO io2{};
   Clue2DParams p2{};
@@ -65,7 +32,8 @@ struct clue_plan {
   hipGraphExec_t exec = nullptr;
 };
 
-namespace {
+namespace clue {
+namespace detail {
 
 int set_err(clue_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -75,12 +43,6 @@ int set_err(clue_ctx* c, int code, const std::string& msg) {
 int hip_fail(clue_ctx* c, hipError_t e, const char* what) {
   return set_err(c, CLUE_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
-
-#define HIP_TRY(ctx, expr)                                   \
-  do {                                                       \
-    hipError_t _e = (expr);                                  \
-    if (_e != hipSuccess) return hip_fail((ctx), _e, #expr); \
-  } while (0)
 
 hipStream_t pick(clue_ctx* c, void* s) { return s ? reinterpret_cast<hipStream_t>(s) : c->stream; }
 
@@ -130,6 +92,13 @@ int claim_scratch(clue_ctx* c, hipStream_t s) {
   return CLUE_OK;
 }
 
+}  // namespace detail
+}  // namespace clue
+
+using namespace clue::detail;
+
+namespace {
+
 bool check_offsets(const int64_t* offs, int n) {
   if (!offs || n < 0 || offs[0] != 0) return false;
   for (int e = 0; e < n; ++e)
@@ -153,52 +122,6 @@ __global__ void k_single_event(int n, int nb, BlockDesc* blocks, int* evOff) {
     evOff[0] = 0;
     evOff[1] = n;
   }
-}
-
-template <class B>
-void carve_common(Carver& cv, B& b, int64_t N, int E, int nBlocks, int64_t stride) {
-  const size_t n = (size_t)(N > 0 ? N : 1);
-  b.E = E;
-  b.N = N;
-  b.nBlocks = nBlocks;
-  b.k1 = cv.take<uint32_t>(n);
-  b.i1 = cv.take<int>(n);
-  b.k2 = cv.take<uint32_t>(n);
-  b.i2 = cv.take<int>(n);
-}
-
-void carve2d(void* base, int64_t N, int E, int nBlocks, Clue2DBuffers& b) {
-  Carver cv{reinterpret_cast<char*>(base)};
-  const size_t n = (size_t)(N > 0 ? N : 1);
-  carve_common(cv, b, N, E, nBlocks, clue2d_table_stride());
-  b.xyw = cv.take<float4>(n);
-  b.xyrp = cv.take<float4>(n);
-  b.hist = cv.take<int>((size_t)kRadix * (nBlocks > 0 ? nBlocks : 1));
-  const int64_t maxScan = (int64_t)kRadix * nBlocks > N ? (int64_t)kRadix * nBlocks : N;
-  b.part = cv.take<int>((size_t)scan_partials(maxScan > 0 ? maxScan : 1));
-  b.tableStride = clue2d_table_stride();
-  b.table = cv.take<int>((size_t)b.tableStride * (E > 0 ? E : 1));
-  b.tileStart = cv.take<int>((size_t)(table_tiles(k2dKeys) + 1) * (E > 0 ? E : 1));
-  b.evOff = cv.take<int>((size_t)E + 1);
-  b.blocks = cv.take<BlockDesc>((size_t)(nBlocks > 0 ? nBlocks : 1));
-}
-
-void carve3d(void* base, int64_t N, int E, int nBlocks, Clue3DBuffers& b) {
-  Carver cv{reinterpret_cast<char*>(base)};
-  const size_t n = (size_t)(N > 0 ? N : 1);
-  carve_common(cv, b, N, E, nBlocks, clue3d_table_stride());
-  b.rozS = cv.take<float>(n);
-  b.phiS = cv.take<float>(n);
-  b.enS = cv.take<float>(n);
-  b.rhoS = cv.take<float>(n);
-  b.hist = cv.take<int>((size_t)kRadix * (nBlocks > 0 ? nBlocks : 1));
-  const int64_t maxScan = (int64_t)kRadix * nBlocks > N ? (int64_t)kRadix * nBlocks : N;
-  b.part = cv.take<int>((size_t)scan_partials(maxScan > 0 ? maxScan : 1));
-  b.tableStride = clue3d_table_stride();
-  b.table = cv.take<int>((size_t)b.tableStride * (E > 0 ? E : 1));
-  b.tileStart = cv.take<int>((size_t)(table_tiles(k3dKeys) + 1) * (E > 0 ? E : 1));
-  b.evOff = cv.take<int>((size_t)E + 1);
-  b.blocks = cv.take<BlockDesc>((size_t)(nBlocks > 0 ? nBlocks : 1));
 }
 
 // Upload a host block table and int32 offsets into the workspace (synchronous copies).
@@ -445,7 +368,7 @@ int32_t clue2d_run_device(clue_ctx* c, const clue2d_params* p, int32_t n, const 
   rc = ensure(c, c->scratch2d, clue2d_workspace_bytes(n, 1, nb));
   if (rc) return rc;
   Clue2DBuffers b;
-  carve2d(c->scratch2d.base, n, 1, nb, b);
+  clue2d_carve(c->scratch2d.base, n, 1, nb, b);
   b.err = c->dErr;
   rc = single_setup(c, n, nb, b.blocks, b.evOff, s);
   if (rc) return rc;
@@ -469,7 +392,7 @@ int32_t clue2d_run_batch_device(clue_ctx* c, const clue2d_params* p, int32_t E, 
   rc = ensure(c, c->scratch2d, clue2d_workspace_bytes(N, E, nb));
   if (rc) return rc;
   Clue2DBuffers b;
-  carve2d(c->scratch2d.base, N, E, nb, b);
+  clue2d_carve(c->scratch2d.base, N, E, nb, b);
   b.err = c->dErr;
   std::vector<BlockDesc> blocks;
   rc = upload_tables(c, offsets, E, b.blocks, b.evOff, blocks);
@@ -533,7 +456,7 @@ clue_plan* clue2d_plan_create(clue_ctx* c, const clue2d_params* p, int32_t E, co
     delete pl;
     return nullptr;
   }
-  carve2d(pl->ws.base, N, E, nb, pl->b2);
+  clue2d_carve(pl->ws.base, N, E, nb, pl->b2);
   pl->b2.err = c->dErr;
   std::vector<BlockDesc> blocks;
   if (upload_tables(c, offsets, E, pl->b2.blocks, pl->b2.evOff, blocks) != CLUE_OK) {
@@ -564,7 +487,7 @@ int32_t clue3d_run_device(clue_ctx* c, const clue3d_params* p, int32_t n, const 
   rc = ensure(c, c->scratch3d, clue3d_workspace_bytes(n, 1, nb));
   if (rc) return rc;
   Clue3DBuffers b;
-  carve3d(c->scratch3d.base, n, 1, nb, b);
+  clue3d_carve(c->scratch3d.base, n, 1, nb, b);
   b.err = c->dErr;
   rc = single_setup(c, n, nb, b.blocks, b.evOff, s);
   if (rc) return rc;
@@ -590,7 +513,7 @@ int32_t clue3d_run_batch_device(clue_ctx* c, const clue3d_params* p, int32_t E, 
   rc = ensure(c, c->scratch3d, clue3d_workspace_bytes(N, E, nb));
   if (rc) return rc;
   Clue3DBuffers b;
-  carve3d(c->scratch3d.base, N, E, nb, b);
+  clue3d_carve(c->scratch3d.base, N, E, nb, b);
   b.err = c->dErr;
   std::vector<BlockDesc> blocks;
   rc = upload_tables(c, offsets, E, b.blocks, b.evOff, blocks);
@@ -650,7 +573,7 @@ clue_plan* clue3d_plan_create(clue_ctx* c, const clue3d_params* p, int32_t E, co
     delete pl;
     return nullptr;
   }
-  carve3d(pl->ws.base, N, E, nb, pl->b3);
+  clue3d_carve(pl->ws.base, N, E, nb, pl->b3);
   pl->b3.err = c->dErr;
   std::vector<BlockDesc> blocks;
   if (upload_tables(c, offsets, E, pl->b3.blocks, pl->b3.evOff, blocks) != CLUE_OK) {

@@ -62,5 +62,6 @@ constexpr int64_t clue2d_table_stride() { return ((int64_t)k2dKeys + 1 + 63) / 6
 
 void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DParams& p, hipStream_t s);
 size_t clue2d_workspace_bytes(int64_t N, int nEvents, int nBlocks);
+void clue2d_carve(void* base, int64_t N, int E, int nBlocks, Clue2DBuffers& b);
 
 }  // namespace clue

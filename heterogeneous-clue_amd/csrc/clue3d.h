@@ -68,5 +68,6 @@ constexpr int64_t clue3d_table_stride() { return ((int64_t)k3dKeys + 1 + 63) / 6
 
 void clue3d_enqueue(const Clue3DBuffers& b, const Clue3DIO& io, const Clue3DParams& p, hipStream_t s);
 size_t clue3d_workspace_bytes(int64_t N, int nEvents, int nBlocks);
+void clue3d_carve(void* base, int64_t N, int E, int nBlocks, Clue3DBuffers& b);
 
 }  // namespace clue

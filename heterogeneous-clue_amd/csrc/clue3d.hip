@@ -285,6 +285,31 @@ void clue3d_enqueue(const Clue3DBuffers& b, const Clue3DIO& io, const Clue3DPara
   launch_labels(b.blocks, b.nBlocks, parent, io.isSeed, rankAll, b.evOff, io.tracksterIndex, s);
 }
 
+// Carve a workspace of clue3d_workspace_bytes(N, E, nBlocks) bytes into the buffers of a batch.
+void clue3d_carve(void* base, int64_t N, int E, int nBlocks, Clue3DBuffers& b) {
+  Carver cv{reinterpret_cast<char*>(base)};
+  const size_t n = (size_t)(N > 0 ? N : 1);
+  b.E = E;
+  b.N = N;
+  b.nBlocks = nBlocks;
+  b.k1 = cv.take<uint32_t>(n);
+  b.i1 = cv.take<int>(n);
+  b.k2 = cv.take<uint32_t>(n);
+  b.i2 = cv.take<int>(n);
+  b.rozS = cv.take<float>(n);
+  b.phiS = cv.take<float>(n);
+  b.enS = cv.take<float>(n);
+  b.rhoS = cv.take<float>(n);
+  b.hist = cv.take<int>((size_t)kRadix * (nBlocks > 0 ? nBlocks : 1));
+  const int64_t maxScan = (int64_t)kRadix * nBlocks > N ? (int64_t)kRadix * nBlocks : N;
+  b.part = cv.take<int>((size_t)scan_partials(maxScan > 0 ? maxScan : 1));
+  b.tableStride = clue3d_table_stride();
+  b.table = cv.take<int>((size_t)b.tableStride * (E > 0 ? E : 1));
+  b.tileStart = cv.take<int>((size_t)(table_tiles(k3dKeys) + 1) * (E > 0 ? E : 1));
+  b.evOff = cv.take<int>((size_t)E + 1);
+  b.blocks = cv.take<BlockDesc>((size_t)(nBlocks > 0 ? nBlocks : 1));
+}
+
 size_t clue3d_workspace_bytes(int64_t N, int nEvents, int nBlocks) {
   const size_t n = (size_t)(N > 0 ? N : 1);
   size_t bytes = 0;

@@ -1,0 +1,469 @@
+// Serving loop: continuous batching of CLUE2D and CLUE3D events on one GPU.
+//
+// Requests (one event each, caller-owned host buffers) are queued by clue_server_submit*. A worker
+// thread owns a fixed set of slots; a slot is one batch in flight with its own HIP stream, pinned
+// staging memory and device workspace, all allocated once at creation against the context's HBM
+// budget and kept resident. Whenever a slot is free and requests are queued, the worker takes the
+// queue head in arrival order (as many as fit the slot: events per batch, points per dimension),
+// packs them into the slot's pinned staging, and enqueues copy-in, the batched kernels and
+// copy-out on the slot's stream. Requests that arrive meanwhile wait for the next free slot, so
+// with two or more slots a new batch starts while earlier ones still run and their work overlaps
+// on the device. When a slot's completion event fires the worker copies each result to its
+// caller's buffers and timestamps it; latency = completion time - submission time.
+//
+// Every event's arithmetic depends only on its own points (blocks never straddle events), so a
+// result is bit-identical whether the event ran alone or in any mix.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "clue2d.h"
+#include "clue3d.h"
+#include "context.h"
+
+using namespace clue;
+using namespace clue::detail;
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+struct Req {
+  int64_t id = 0;
+  int dim = 2;
+  int32_t n = 0;
+  const void* in[8] = {};
+  void* out[7] = {};
+  Clock::time_point tSubmit, tDone;
+  int state = 0;  // 0 queued, 1 running, 2 done, < 0 error code
+};
+
+constexpr int kIn2 = 4, kOut2 = 5, kIn3 = 8, kOut3 = 7;
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  Workspace ws2, ws3, io2, io3;  // device: batch workspaces and I/O SoA (capacity-sized)
+  char* stage = nullptr;         // pinned host staging
+  size_t stageBytes = 0;
+  std::vector<Req*> reqs2, reqs3;
+  std::vector<int64_t> offs2, offs3;
+  bool busy = false;
+};
+
+}  // namespace
+
+struct clue_server {
+  clue_ctx* ctx = nullptr;
+  Clue2DParams p2{};
+  Clue3DParams p3{};
+  int64_t cap2 = 0, cap3 = 0;
+  int capEvents = 1;
+  std::vector<Slot> slots;
+  std::mutex m;
+  std::condition_variable cvWork, cvDone;
+  std::deque<Req*> queue;
+  std::unordered_map<int64_t, std::unique_ptr<Req>> reqs;
+  std::thread worker;
+  bool stop = false;
+  int64_t nextId = 0;
+  int64_t nBatches = 0, nEvents = 0, nPoints = 0;
+  std::string err;
+};
+
+namespace {
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Pinned staging layout of a slot: 2D inputs and outputs, 3D inputs and outputs (field-major,
+// capacity-sized), then block tables and offsets for both dimensions.
+struct StageLayout {
+  size_t in2, out2, in3, out3, blk2, off2, blk3, off3, total;
+};
+
+StageLayout stage_layout(int64_t cap2, int64_t cap3, int capEvents) {
+  StageLayout L;
+  size_t p = 0;
+  L.in2 = p;
+  p += align256((size_t)cap2 * 4) * kIn2;
+  L.out2 = p;
+  p += align256((size_t)cap2 * 4) * kOut2;
+  L.in3 = p;
+  p += align256((size_t)cap3 * 4) * kIn3;
+  L.out3 = p;
+  p += align256((size_t)cap3 * 4) * kOut3;
+  const size_t maxBlk2 = (size_t)(cap2 / kSortBlock + capEvents + 1);
+  const size_t maxBlk3 = (size_t)(cap3 / kSortBlock + capEvents + 1);
+  L.blk2 = p;
+  p += align256(maxBlk2 * sizeof(BlockDesc));
+  L.off2 = p;
+  p += align256((size_t)(capEvents + 1) * 4);
+  L.blk3 = p;
+  p += align256(maxBlk3 * sizeof(BlockDesc));
+  L.off3 = p;
+  p += align256((size_t)(capEvents + 1) * 4);
+  L.total = p;
+  return L;
+}
+
+int server_fail(clue_server* s, int code, const std::string& msg) {
+  std::lock_guard<std::mutex> g(s->m);
+  s->err = msg;
+  return code;
+}
+
+// Pack and launch the requests assigned to a slot. Returns CLUE_OK or an error code.
+int launch_slot(clue_server* s, Slot& sl) {
+  clue_ctx* c = s->ctx;
+  const StageLayout L = stage_layout(s->cap2, s->cap3, s->capEvents);
+  const size_t f2 = align256((size_t)s->cap2 * 4), f3 = align256((size_t)s->cap3 * 4);
+  char* st = sl.stage;
+  // ---- 2D
+  if (!sl.reqs2.empty()) {
+    const int E = (int)sl.reqs2.size();
+    sl.offs2.assign((size_t)E + 1, 0);
+    for (int e = 0; e < E; ++e) sl.offs2[e + 1] = sl.offs2[e] + sl.reqs2[e]->n;
+    const int64_t N = sl.offs2[E];
+    for (int e = 0; e < E; ++e)
+      for (int k = 0; k < kIn2; ++k)
+        std::memcpy(st + L.in2 + k * f2 + (size_t)sl.offs2[e] * 4, sl.reqs2[e]->in[k], (size_t)sl.reqs2[e]->n * 4);
+    char* din = reinterpret_cast<char*>(sl.io2.base);
+    for (int k = 0; k < kIn2; ++k)
+      HIP_TRY(c, hipMemcpyAsync(din + k * f2, st + L.in2 + k * f2, (size_t)N * 4, hipMemcpyHostToDevice, sl.stream));
+    const int nb = [&] {
+      int64_t b = 0;
+      for (int e = 0; e < E; ++e) b += (sl.offs2[e + 1] - sl.offs2[e] + kSortBlock - 1) / kSortBlock;
+      return (int)b;
+    }();
+    Clue2DBuffers b;
+    clue2d_carve(sl.ws2.base, N, E, nb, b);
+    b.err = c->dErr;
+    const std::vector<BlockDesc> blocks = make_blocks(sl.offs2.data(), E);
+    std::memcpy(st + L.blk2, blocks.data(), blocks.size() * sizeof(BlockDesc));
+    int* off32 = reinterpret_cast<int*>(st + L.off2);
+    for (int e = 0; e <= E; ++e) off32[e] = (int)sl.offs2[e];
+    HIP_TRY(c, hipMemcpyAsync(const_cast<BlockDesc*>(b.blocks), st + L.blk2, blocks.size() * sizeof(BlockDesc),
+                              hipMemcpyHostToDevice, sl.stream));
+    HIP_TRY(c, hipMemcpyAsync(const_cast<int*>(b.evOff), off32, (size_t)(E + 1) * 4, hipMemcpyHostToDevice, sl.stream));
+    char* dout = din + kIn2 * f2;
+    Clue2DIO io{reinterpret_cast<float*>(din), reinterpret_cast<float*>(din + f2),
+                reinterpret_cast<int32_t*>(din + 2 * f2), reinterpret_cast<float*>(din + 3 * f2),
+                reinterpret_cast<float*>(dout), reinterpret_cast<float*>(dout + f2),
+                reinterpret_cast<int32_t*>(dout + 2 * f2), reinterpret_cast<int32_t*>(dout + 3 * f2),
+                reinterpret_cast<int32_t*>(dout + 4 * f2)};
+    clue2d_enqueue(b, io, s->p2, sl.stream);
+    HIP_TRY(c, hipGetLastError());
+    for (int k = 0; k < kOut2; ++k)
+      HIP_TRY(c, hipMemcpyAsync(st + L.out2 + k * f2, dout + k * f2, (size_t)N * 4, hipMemcpyDeviceToHost, sl.stream));
+  }
+  // ---- 3D
+  if (!sl.reqs3.empty()) {
+    const int E = (int)sl.reqs3.size();
+    sl.offs3.assign((size_t)E + 1, 0);
+    for (int e = 0; e < E; ++e) sl.offs3[e + 1] = sl.offs3[e] + sl.reqs3[e]->n;
+    const int64_t N = sl.offs3[E];
+    for (int e = 0; e < E; ++e)
+      for (int k = 0; k < kIn3; ++k)
+        std::memcpy(st + L.in3 + k * f3 + (size_t)sl.offs3[e] * 4, sl.reqs3[e]->in[k], (size_t)sl.reqs3[e]->n * 4);
+    char* din = reinterpret_cast<char*>(sl.io3.base);
+    for (int k = 0; k < kIn3; ++k)
+      HIP_TRY(c, hipMemcpyAsync(din + k * f3, st + L.in3 + k * f3, (size_t)N * 4, hipMemcpyHostToDevice, sl.stream));
+    int64_t nb64 = 0;
+    for (int e = 0; e < E; ++e) nb64 += (sl.offs3[e + 1] - sl.offs3[e] + kSortBlock - 1) / kSortBlock;
+    const int nb = (int)nb64;
+    Clue3DBuffers b;
+    clue3d_carve(sl.ws3.base, N, E, nb, b);
+    b.err = c->dErr;
+    const std::vector<BlockDesc> blocks = make_blocks(sl.offs3.data(), E);
+    std::memcpy(st + L.blk3, blocks.data(), blocks.size() * sizeof(BlockDesc));
+    int* off32 = reinterpret_cast<int*>(st + L.off3);
+    for (int e = 0; e <= E; ++e) off32[e] = (int)sl.offs3[e];
+    HIP_TRY(c, hipMemcpyAsync(const_cast<BlockDesc*>(b.blocks), st + L.blk3, blocks.size() * sizeof(BlockDesc),
+                              hipMemcpyHostToDevice, sl.stream));
+    HIP_TRY(c, hipMemcpyAsync(const_cast<int*>(b.evOff), off32, (size_t)(E + 1) * 4, hipMemcpyHostToDevice, sl.stream));
+    char* dout = din + kIn3 * f3;
+    auto F = [&](char* base, int k) { return reinterpret_cast<float*>(base + k * f3); };
+    auto I = [&](char* base, int k) { return reinterpret_cast<int32_t*>(base + k * f3); };
+    Clue3DIO io{F(din, 0), F(din, 1), F(din, 2), F(din, 3), F(din, 4), F(din, 5), I(din, 6), I(din, 7),
+                F(dout, 0), F(dout, 1), I(dout, 2), I(dout, 3), I(dout, 4), I(dout, 5), I(dout, 6)};
+    clue3d_enqueue(b, io, s->p3, sl.stream);
+    HIP_TRY(c, hipGetLastError());
+    for (int k = 0; k < kOut3; ++k)
+      HIP_TRY(c, hipMemcpyAsync(st + L.out3 + k * f3, dout + k * f3, (size_t)N * 4, hipMemcpyDeviceToHost, sl.stream));
+  }
+  HIP_TRY(c, hipEventRecord(sl.done, sl.stream));
+  return CLUE_OK;
+}
+
+// Copy the results of a finished slot to the callers' buffers and mark the requests done.
+void finish_slot(clue_server* s, Slot& sl, int status) {
+  const StageLayout L = stage_layout(s->cap2, s->cap3, s->capEvents);
+  const size_t f2 = align256((size_t)s->cap2 * 4), f3 = align256((size_t)s->cap3 * 4);
+  if (status == CLUE_OK) {
+    for (size_t e = 0; e < sl.reqs2.size(); ++e)
+      for (int k = 0; k < kOut2; ++k)
+        std::memcpy(sl.reqs2[e]->out[k], sl.stage + L.out2 + k * f2 + (size_t)sl.offs2[e] * 4,
+                    (size_t)sl.reqs2[e]->n * 4);
+    for (size_t e = 0; e < sl.reqs3.size(); ++e)
+      for (int k = 0; k < kOut3; ++k)
+        std::memcpy(sl.reqs3[e]->out[k], sl.stage + L.out3 + k * f3 + (size_t)sl.offs3[e] * 4,
+                    (size_t)sl.reqs3[e]->n * 4);
+  }
+  const auto now = Clock::now();
+  {
+    std::lock_guard<std::mutex> g(s->m);
+    for (Req* r : sl.reqs2) {
+      r->tDone = now;
+      r->state = status == CLUE_OK ? 2 : status;
+    }
+    for (Req* r : sl.reqs3) {
+      r->tDone = now;
+      r->state = status == CLUE_OK ? 2 : status;
+    }
+  }
+  sl.reqs2.clear();
+  sl.reqs3.clear();
+  sl.busy = false;
+  s->cvDone.notify_all();
+}
+
+void worker_loop(clue_server* s) {
+  (void)hipSetDevice(s->ctx->device);
+  for (;;) {
+    bool active = false;
+    // completions
+    for (auto& sl : s->slots) {
+      if (!sl.busy) continue;
+      const hipError_t q = hipEventQuery(sl.done);
+      if (q == hipSuccess) {
+        finish_slot(s, sl, CLUE_OK);
+      } else if (q != hipErrorNotReady) {
+        server_fail(s, CLUE_ERR_HIP, std::string("batch failed: ") + hipGetErrorString(q));
+        finish_slot(s, sl, CLUE_ERR_HIP);
+      } else {
+        active = true;
+      }
+    }
+    // admissions: the queue head, in arrival order, into a free slot
+    Slot* free = nullptr;
+    for (auto& sl : s->slots)
+      if (!sl.busy) {
+        free = &sl;
+        break;
+      }
+    if (free) {
+      std::unique_lock<std::mutex> lk(s->m);
+      if (s->stop && s->queue.empty()) break;
+      int64_t used2 = 0, used3 = 0;
+      while (!s->queue.empty()) {
+        Req* r = s->queue.front();
+        std::vector<Req*>& dst = r->dim == 2 ? free->reqs2 : free->reqs3;
+        int64_t& used = r->dim == 2 ? used2 : used3;
+        const int64_t cap = r->dim == 2 ? s->cap2 : s->cap3;
+        if ((int)(free->reqs2.size() + free->reqs3.size()) >= s->capEvents || used + r->n > cap) break;
+        used += r->n;
+        dst.push_back(r);
+        r->state = 1;
+        s->queue.pop_front();
+      }
+      const bool any = !free->reqs2.empty() || !free->reqs3.empty();
+      if (any) {
+        s->nBatches += 1;
+        s->nEvents += (int64_t)(free->reqs2.size() + free->reqs3.size());
+        s->nPoints += used2 + used3;
+      }
+      lk.unlock();
+      if (any) {
+        free->busy = true;
+        const int rc = launch_slot(s, *free);
+        if (rc != CLUE_OK) {
+          server_fail(s, rc, s->ctx->err);
+          (void)hipStreamSynchronize(free->stream);
+          finish_slot(s, *free, rc);
+        }
+        continue;
+      }
+    }
+    if (active) {
+      std::this_thread::sleep_for(std::chrono::microseconds(10));
+    } else {
+      std::unique_lock<std::mutex> lk(s->m);
+      if (s->stop && s->queue.empty()) break;
+      s->cvWork.wait_for(lk, std::chrono::milliseconds(1), [&] { return s->stop || !s->queue.empty(); });
+    }
+  }
+}
+
+int64_t submit(clue_server* s, int dim, int32_t n, const void* const* in, void* const* out) {
+  if (!s || n < 0) return CLUE_ERR_ARG;
+  const int nin = dim == 2 ? kIn2 : kIn3, nout = dim == 2 ? kOut2 : kOut3;
+  for (int k = 0; k < nin; ++k)
+    if (n > 0 && !in[k]) return server_fail(s, CLUE_ERR_ARG, "null input buffer");
+  for (int k = 0; k < nout; ++k)
+    if (n > 0 && !out[k]) return server_fail(s, CLUE_ERR_ARG, "null output buffer");
+  if (n > (dim == 2 ? s->cap2 : s->cap3))
+    return server_fail(s, CLUE_ERR_BUDGET, "event larger than the server's per-batch capacity");
+  // Layers are checked here so that a bad event is rejected alone instead of failing its batch.
+  const int32_t* layer = static_cast<const int32_t*>(in[dim == 2 ? 2 : 6]);
+  const int nLayers = dim == 2 ? k2dLayers : k3dLayers;
+  for (int32_t i = 0; i < n; ++i)
+    if (layer[i] < 0 || layer[i] >= nLayers) return server_fail(s, CLUE_ERR_LAYER, "input layer outside the tile array");
+  auto r = std::make_unique<Req>();
+  r->dim = dim;
+  r->n = n;
+  for (int k = 0; k < nin; ++k) r->in[k] = in[k];
+  for (int k = 0; k < nout; ++k) r->out[k] = out[k];
+  r->tSubmit = Clock::now();
+  int64_t id;
+  {
+    std::lock_guard<std::mutex> g(s->m);
+    id = s->nextId++;
+    r->id = id;
+    if (n == 0) {
+      r->state = 2;
+      r->tDone = r->tSubmit;
+    } else {
+      s->queue.push_back(r.get());
+    }
+    s->reqs.emplace(id, std::move(r));
+  }
+  s->cvWork.notify_one();
+  return id;
+}
+
+}  // namespace
+
+extern "C" {
+
+clue_server* clue_server_create(clue_ctx* c, const clue2d_params* p2, const clue3d_params* p3,
+                                int64_t max_points_2d, int64_t max_points_3d, int32_t max_events, int32_t slots) {
+  if (!c || !p2 || !p3 || max_points_2d < 1 || max_points_3d < 1 || max_events < 1 || slots < 1 ||
+      max_points_2d >= (int64_t)0x7fffffff || max_points_3d >= (int64_t)0x7fffffff) {
+    if (c) set_err(c, CLUE_ERR_ARG, "bad server configuration");
+    return nullptr;
+  }
+  auto s = std::make_unique<clue_server>();
+  s->ctx = c;
+  s->p2 = make_clue2d_params(p2->dc, p2->rhoc, p2->outlierDeltaFactor);
+  s->p3 = Clue3DParams{p3->densitySiblingLayers, p3->densityXYDistanceSqr, p3->kernelDensityFactor,
+                       p3->densityOnSameLayer, p3->nearestHigherOnSameLayer, p3->criticalXYDistance,
+                       p3->criticalZDistanceLyr, p3->criticalDensity, p3->criticalSelfDensity,
+                       p3->outlierMultiplier};
+  s->cap2 = max_points_2d;
+  s->cap3 = max_points_3d;
+  s->capEvents = max_events;
+  s->slots.resize((size_t)slots);
+  const StageLayout L = stage_layout(s->cap2, s->cap3, s->capEvents);
+  const int nb2 = (int)(s->cap2 / kSortBlock + s->capEvents + 1), nb3 = (int)(s->cap3 / kSortBlock + s->capEvents + 1);
+  bool ok = true;
+  for (auto& sl : s->slots) {
+    ok = ok && hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess;
+    ok = ok && ensure(c, sl.ws2, clue2d_workspace_bytes(s->cap2, s->capEvents, nb2)) == CLUE_OK;
+    ok = ok && ensure(c, sl.ws3, clue3d_workspace_bytes(s->cap3, s->capEvents, nb3)) == CLUE_OK;
+    ok = ok && ensure(c, sl.io2, align256((size_t)s->cap2 * 4) * (kIn2 + kOut2)) == CLUE_OK;
+    ok = ok && ensure(c, sl.io3, align256((size_t)s->cap3 * 4) * (kIn3 + kOut3)) == CLUE_OK;
+    if (ok) {
+      void* p = nullptr;
+      ok = hipHostMalloc(&p, L.total, hipHostMallocDefault) == hipSuccess;
+      sl.stage = static_cast<char*>(p);
+      sl.stageBytes = L.total;
+    }
+    if (!ok) break;
+  }
+  if (!ok) {
+    for (auto& sl : s->slots) {
+      if (sl.stage) (void)hipHostFree(sl.stage);
+      for (Workspace* w : {&sl.ws2, &sl.ws3, &sl.io2, &sl.io3})
+        if (w->base) dev_free(c, w->base);
+      if (sl.done) (void)hipEventDestroy(sl.done);
+      if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    }
+    if (c->err.empty()) set_err(c, CLUE_ERR_HIP, "server resources could not be created");
+    return nullptr;
+  }
+  clue_server* raw = s.release();
+  raw->worker = std::thread(worker_loop, raw);
+  return raw;
+}
+
+void clue_server_destroy(clue_server* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> g(s->m);
+    s->stop = true;
+  }
+  s->cvWork.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  for (auto& sl : s->slots) {
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    if (sl.stage) (void)hipHostFree(sl.stage);
+    for (Workspace* w : {&sl.ws2, &sl.ws3, &sl.io2, &sl.io3})
+      if (w->base) dev_free(s->ctx, w->base);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+  }
+  delete s;
+}
+
+int64_t clue_server_submit2d(clue_server* s, int32_t n, const float* x, const float* y, const int32_t* layer,
+                             const float* weight, float* rho, float* delta, int32_t* nearestHigher, int32_t* isSeed,
+                             int32_t* clusterIndex) {
+  const void* in[kIn2] = {x, y, layer, weight};
+  void* out[kOut2] = {rho, delta, nearestHigher, isSeed, clusterIndex};
+  return submit(s, 2, n, in, out);
+}
+
+int64_t clue_server_submit3d(clue_server* s, int32_t n, const float* z, const float* eta, const float* phi,
+                             const float* r_over_absz, const float* radius, const float* energy, const int32_t* layer,
+                             const int32_t* isSilicon, float* rho, float* delta_first, int32_t* delta_second,
+                             int32_t* nh_layer, int32_t* nh_index, int32_t* isSeed, int32_t* tracksterIndex) {
+  const void* in[kIn3] = {z, eta, phi, r_over_absz, radius, energy, layer, isSilicon};
+  void* out[kOut3] = {rho, delta_first, delta_second, nh_layer, nh_index, isSeed, tracksterIndex};
+  return submit(s, 3, n, in, out);
+}
+
+int32_t clue_server_wait(clue_server* s, int64_t id, int32_t timeout_ms, double* latency_ms) {
+  if (!s) return CLUE_ERR_ARG;
+  std::unique_lock<std::mutex> lk(s->m);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) {
+    s->err = "unknown request id";
+    return CLUE_ERR_ARG;
+  }
+  Req* r = it->second.get();
+  const bool done = s->cvDone.wait_for(lk, std::chrono::milliseconds(timeout_ms < 0 ? 0 : timeout_ms),
+                                       [&] { return r->state == 2 || r->state < 0; });
+  if (!done) return CLUE_ERR_TIMEOUT;
+  const int state = r->state;
+  if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->tDone - r->tSubmit).count();
+  s->reqs.erase(it);
+  return state == 2 ? CLUE_OK : state;
+}
+
+int32_t clue_server_stats(clue_server* s, int64_t* batches, int64_t* events, int64_t* points) {
+  if (!s) return CLUE_ERR_ARG;
+  std::lock_guard<std::mutex> g(s->m);
+  if (batches) *batches = s->nBatches;
+  if (events) *events = s->nEvents;
+  if (points) *points = s->nPoints;
+  return CLUE_OK;
+}
+
+const char* clue_server_last_error(clue_server* s) {
+  if (!s) return "null server";
+  std::lock_guard<std::mutex> g(s->m);
+  thread_local std::string copy;
+  copy = s->err;
+  return copy.c_str();
+}
+
+}  // extern "C"

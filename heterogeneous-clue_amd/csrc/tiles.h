@@ -137,6 +137,17 @@ void build_table(const uint32_t* keySorted, const BlockDesc* blocksDev, int nBlo
 void launch_labels(const BlockDesc* blocksDev, int nBlocks, int* parent, const int* isSeed, const int* rankAll,
                    const int* evOffDev, int32_t* label, hipStream_t s);
 
+// Sequential carving of one allocation into 256-byte aligned sub-buffers.
+struct Carver {
+  char* p;
+  template <class T>
+  T* take(size_t count) {
+    T* r = reinterpret_cast<T*>(p);
+    p += (count * sizeof(T) + 255) & ~(size_t)255;
+    return r;
+  }
+};
+
 // Fill an int buffer on the stream with a kernel (graph-capture friendly, no memset nodes).
 void fill_i32(int* p, int value, int64_t n, hipStream_t s);
 

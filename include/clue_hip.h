@@ -32,9 +32,11 @@ extern "C" {
 #define CLUE_ERR_BUDGET (-3)   /* the allocation would exceed the context's HBM budget */
 #define CLUE_ERR_LAYER (-4)    /* an input layer outside the tile array (2D: [0,100), 3D: [0,94)) */
 #define CLUE_ERR_BIN (-5)      /* CLUE3D: a phi so far outside [-pi, pi] that its bin leaves the tile */
+#define CLUE_ERR_TIMEOUT (-6)  /* clue_server_wait: the request did not finish in time */
 
 typedef struct clue_ctx clue_ctx;
 typedef struct clue_plan clue_plan;
+typedef struct clue_server clue_server;
 
 /* CLUE2D parameters: Parameters {dc, rhoc, outlierDeltaFactor} (src/serial/DataFormats/CLUE_config.h:8-13),
  * produced by CLUESerialClusterizerESProducer from the CSV `dc,rhoc,odf,produceOutput`
@@ -162,6 +164,35 @@ clue_plan* clue3d_plan_create(clue_ctx* ctx, const clue3d_params* p, int32_t n_e
 int32_t clue_plan_launch(clue_plan* plan, void* stream);
 void clue_plan_destroy(clue_plan* plan);
 uint64_t clue_plan_workspace_bytes(const clue_plan* plan);
+
+/* ---------------------------------------------------------------- serving loop
+ * No reference counterpart: the reference processes one event per produce call. A server owns
+ * `slots` batches in flight, each with its own HIP stream, pinned staging and device workspace for
+ * up to `max_events` events and `max_points_2d` / `max_points_3d` points, allocated once against
+ * the context's HBM budget and kept resident. Submitted events queue in arrival order; whenever a
+ * slot is free, a worker thread takes as many queued events as fit into it, runs them as one batch
+ * and copies each result into the caller's output buffers (continuous batching: events arriving
+ * while batches run are admitted as slots free up, and batches on different slots overlap on the
+ * device). Input and output buffers are the caller's and must stay valid until clue_server_wait
+ * returns for the request. While a server exists, its context must not be used from other threads.
+ * A result is bit-identical to the same event run alone. */
+clue_server* clue_server_create(clue_ctx* ctx, const clue2d_params* p2, const clue3d_params* p3,
+                                int64_t max_points_2d, int64_t max_points_3d, int32_t max_events, int32_t slots);
+void clue_server_destroy(clue_server* server);
+/* Returns a request id >= 0, or a negative error code (e.g. CLUE_ERR_LAYER, CLUE_ERR_BUDGET when the
+ * event exceeds a slot's capacity). */
+int64_t clue_server_submit2d(clue_server* server, int32_t n, const float* x, const float* y, const int32_t* layer,
+                             const float* weight, float* rho, float* delta, int32_t* nearestHigher, int32_t* isSeed,
+                             int32_t* clusterIndex);
+int64_t clue_server_submit3d(clue_server* server, int32_t n, const float* z, const float* eta, const float* phi,
+                             const float* r_over_absz, const float* radius, const float* energy, const int32_t* layer,
+                             const int32_t* isSilicon, float* rho, float* delta_first, int32_t* delta_second,
+                             int32_t* nh_layer, int32_t* nh_index, int32_t* isSeed, int32_t* tracksterIndex);
+/* Blocks until request `id` is done (outputs written) or timeout_ms elapses (CLUE_ERR_TIMEOUT);
+ * on success stores the request's latency (submission to results in the caller's buffers). */
+int32_t clue_server_wait(clue_server* server, int64_t id, int32_t timeout_ms, double* latency_ms);
+int32_t clue_server_stats(clue_server* server, int64_t* batches, int64_t* events, int64_t* points);
+const char* clue_server_last_error(clue_server* server);
 
 /* ---------------------------------------------------------------- arithmetic self-check
  * Evaluates the device binning and distance helpers on n host values (synchronous), so that the
