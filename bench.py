@@ -88,6 +88,55 @@ def cpu_baseline(packed, offs, p, gpu_out):
             "seconds": round(t, 3), "labels_bit_exact_vs_gpu": bool(same)}
 
 
+def serving_run(dev, pool, order, arrivals, slots, max_events, p3):
+    """Open-loop Poisson stream: submit each event at its arrival time, wait for all."""
+    srv = clue_hip.Server(dev, p3=p3, max_points_2d=4_000_000, max_points_3d=1_000_000, max_events=max_events,
+                          slots=slots)
+    ids = []
+    t0 = time.perf_counter()
+    for k, t in zip(order, arrivals):
+        while time.perf_counter() - t0 < t:
+            pass
+        d, ev = pool[k]
+        ids.append(srv.submit2d(ev) if d == 2 else srv.submit3d(ev))
+    lat = np.array([srv.wait(i, 120_000)[1] for i in ids])
+    span = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    return lat, span, st
+
+
+def serving_leg(dev, rate, seconds, seed=11):  # noqa: C901
+    """C5: events arrive as a Poisson stream, 70 % CLUE2D (1e3-1e6 hits, log-uniform) and 30 % CLUE3D
+    (50 layers x Poisson(200) layer clusters); the continuous-batching server (3 slots, up to 64 events
+    per batch) and per-event processing (1 slot, 1 event per batch) see the same stream."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n2 = synth.c3_hit_counts(96, seed=seed)
+    pool = [(2, synth.clue2d_event(int(n), 5000 + k)) for k, n in enumerate(n2)]
+    pool += [(3, synth.clue3d_event(6000 + k)) for k in range(48)]
+    # events wait in pinned host memory, as a front end receiving them by DMA would hold them
+    pool = [(d, {k: dev.pinned_copy(v) for k, v in ev.items()}) for d, ev in pool]
+    idx2 = [k for k, (d, _) in enumerate(pool) if d == 2]
+    idx3 = [k for k, (d, _) in enumerate(pool) if d == 3]
+    n_events = int(rate * seconds)
+    order = [int(rng.choice(idx2)) if rng.random() < 0.7 else int(rng.choice(idx3)) for _ in range(n_events)]
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_events))
+    hits = sum(pool[k][1]["layer"].size for k in order)
+    p3 = clue_hip.params3d(S=2)
+    out = {"rate_events_per_s": rate, "events": n_events, "hits": int(hits),
+           "mix": "70% CLUE2D 1e3-1e6 hits log-uniform, 30% CLUE3D 50 layers x Poisson(200)",
+           "latency": "submission to results in the caller's host buffers (H2D, kernels, D2H included)"}
+    for name, slots, maxev in (("batched", 3, 64), ("per_event", 1, 1)):
+        serving_run(dev, pool, order[:50], arrivals[:50] * 0.0, slots, maxev, p3)  # warm the slots
+        lat, span, st = serving_run(dev, pool, order, arrivals, slots, maxev, p3)
+        out[name] = {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                     "mean_ms": round(float(lat.mean()), 3), "mhits_per_s": round(hits / span / 1e6, 1),
+                     "batches": st["batches"], "slots": slots, "max_events_per_batch": maxev}
+        log(f"serving {name} at {rate}/s: p50 {out[name]['p50_ms']} ms p99 {out[name]['p99_ms']} ms, "
+            f"{st['batches']} batches")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +146,9 @@ def main():
     ap.add_argument("--per-event-reps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-per-event", action="store_true", help="skip the per-event comparison leg")
+    ap.add_argument("--serve-rates", default="500,1500", help="serving leg: event rates (events/s), comma-separated")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="serving leg: length of the stream")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); events are independent, so N GPUs "
@@ -156,6 +208,9 @@ def main():
                                "labels_bit_exact_vs_batched": bool(same)}
         result["batched_over_per_event"] = round(value / pe, 2)
         log(f"per-event: {t * 1e3:.2f} ms total ({pe:.1f} Mhits/s), bit-exact={same}")
+
+    if not args.no_serve:
+        result["serving"] = [serving_leg(dev, float(r), args.serve_seconds) for r in args.serve_rates.split(",")]
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(packed, offs, p, batch_out)

@@ -169,8 +169,25 @@ class Device:
 
     def close(self):
         if self.ctx:
+            for ptr in getattr(self, "_pinned", []):
+                self.lib.clue_free_pinned(self.ctx, ptr)
+            self._pinned = []
             self.lib.clue_destroy(self.ctx)
             self.ctx = None
+
+    def pinned_copy(self, a: np.ndarray) -> np.ndarray:
+        """A numpy array in pinned host memory (freed with the device) holding a copy of `a`."""
+        a = np.ascontiguousarray(a)
+        ptr = self.lib.clue_malloc_pinned(self.ctx, max(a.nbytes, 4))
+        if not ptr:
+            raise ClueError(-2, self.error())
+        if not hasattr(self, "_pinned"):
+            self._pinned = []
+        self._pinned.append(ptr)
+        buf = (C.c_char * max(a.nbytes, 4)).from_address(ptr)
+        out = np.frombuffer(buf, dtype=a.dtype, count=a.size)
+        out[...] = a
+        return out
 
     def error(self) -> str:
         return self.lib.clue_last_error(self.ctx).decode()

@@ -43,8 +43,97 @@ struct Req {
   const void* in[8] = {};
   void* out[7] = {};
   Clock::time_point tSubmit, tDone;
-  int state = 0;  // 0 queued, 1 running, 2 done, < 0 error code
+  int state = 0;          // 0 queued, 1 running, 2 done, < 0 error code
+  bool pinnedIn = false;  // every input buffer is pinned host memory: copied by DMA directly
+  bool pinnedOut = false;  // every output buffer is pinned host memory
 };
+
+// Host memcpys of a batch (pageable caller buffers <-> pinned staging), split into pieces of at
+// most kPiece bytes and run by a few threads plus the caller. add() queues copies; run_all() arms
+// the pool, helps, and returns once every queued copy is done. Only one thread (the server's
+// worker) uses a pool.
+class CopyPool {
+ public:
+  explicit CopyPool(int threads) {
+    for (int t = 0; t < threads; ++t) workers_.emplace_back([this] { run(); });
+  }
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> g(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& w : workers_) w.join();
+  }
+  void add(void* dst, const void* src, size_t bytes) {
+    std::lock_guard<std::mutex> g(m_);
+    for (size_t o = 0; o < bytes; o += kPiece)
+      tasks_.push_back({static_cast<char*>(dst) + o, static_cast<const char*>(src) + o,
+                        bytes - o < kPiece ? bytes - o : kPiece});
+  }
+  void run_all() {
+    {
+      std::lock_guard<std::mutex> g(m_);
+      if (tasks_.empty()) return;
+      next_ = 0;
+      pending_ = tasks_.size();
+      armed_ = true;
+    }
+    cv_.notify_all();
+    work();
+    std::unique_lock<std::mutex> lk(m_);
+    done_.wait(lk, [&] { return pending_ == 0; });
+    armed_ = false;
+    tasks_.clear();
+  }
+
+ private:
+  struct Task {
+    char* dst;
+    const char* src;
+    size_t bytes;
+  };
+  static constexpr size_t kPiece = 1u << 22;
+  // Takes and runs tasks until none is left to take.
+  void work() {
+    for (;;) {
+      Task t;
+      {
+        std::lock_guard<std::mutex> g(m_);
+        if (!armed_ || next_ >= tasks_.size()) return;
+        t = tasks_[next_++];
+      }
+      std::memcpy(t.dst, t.src, t.bytes);
+      std::lock_guard<std::mutex> g(m_);
+      if (--pending_ == 0) done_.notify_all();
+    }
+  }
+  void run() {
+    std::unique_lock<std::mutex> lk(m_);
+    for (;;) {
+      cv_.wait(lk, [&] { return stop_ || (armed_ && next_ < tasks_.size()); });
+      if (stop_) return;
+      lk.unlock();
+      work();
+      lk.lock();
+    }
+  }
+  std::vector<std::thread> workers_;
+  std::vector<Task> tasks_;
+  std::mutex m_;
+  std::condition_variable cv_, done_;
+  size_t next_ = 0, pending_ = 0;
+  bool armed_ = false, stop_ = false;
+};
+
+bool is_pinned(const void* p) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();  // pageable memory is reported as an error by some runtimes
+    return false;
+  }
+  return a.type == hipMemoryTypeHost;
+}
 
 constexpr int kIn2 = 4, kOut2 = 5, kIn3 = 8, kOut3 = 7;
 
@@ -77,6 +166,7 @@ struct clue_server {
   int64_t nextId = 0;
   int64_t nBatches = 0, nEvents = 0, nPoints = 0;
   std::string err;
+  std::unique_ptr<CopyPool> copies;
 };
 
 namespace {
@@ -120,6 +210,65 @@ int server_fail(clue_server* s, int code, const std::string& msg) {
   return code;
 }
 
+// Inputs of a batch into the slot's device SoA (field k at dev + k*stride, event e at offs[e]):
+// pinned caller buffers by DMA directly; pageable ones through the pinned staging (parallel host
+// copies) and one DMA per contiguous staged run.
+hipError_t copy_in(clue_server* s, Slot& sl, const std::vector<Req*>& reqs, const std::vector<int64_t>& offs,
+                   int nf, char* stage, char* dev, size_t stride) {
+  bool staged = false;
+  for (size_t e = 0; e < reqs.size(); ++e)
+    if (!reqs[e]->pinnedIn) {
+      staged = true;
+      for (int k = 0; k < nf; ++k)
+        s->copies->add(stage + k * stride + (size_t)offs[e] * 4, reqs[e]->in[k], (size_t)reqs[e]->n * 4);
+    }
+  if (staged) s->copies->run_all();
+  for (int k = 0; k < nf; ++k) {
+    size_t e = 0;
+    while (e < reqs.size()) {
+      if (reqs[e]->pinnedIn) {
+        hipError_t r = hipMemcpyAsync(dev + k * stride + (size_t)offs[e] * 4, reqs[e]->in[k], (size_t)reqs[e]->n * 4,
+                                      hipMemcpyHostToDevice, sl.stream);
+        if (r != hipSuccess) return r;
+        ++e;
+        continue;
+      }
+      size_t f = e;
+      while (f < reqs.size() && !reqs[f]->pinnedIn) ++f;  // a run of staged events
+      const size_t a = (size_t)offs[e] * 4, b = (size_t)offs[f] * 4;
+      hipError_t r = hipMemcpyAsync(dev + k * stride + a, stage + k * stride + a, b - a, hipMemcpyHostToDevice, sl.stream);
+      if (r != hipSuccess) return r;
+      e = f;
+    }
+  }
+  return hipSuccess;
+}
+
+// Outputs of a batch: into pinned caller buffers directly, otherwise into the staging (copied to
+// the caller by finish_slot).
+hipError_t copy_out(Slot& sl, const std::vector<Req*>& reqs, const std::vector<int64_t>& offs, int nf, char* stage,
+                    char* dev, size_t stride) {
+  for (int k = 0; k < nf; ++k) {
+    size_t e = 0;
+    while (e < reqs.size()) {
+      if (reqs[e]->pinnedOut) {
+        hipError_t r = hipMemcpyAsync(reqs[e]->out[k], dev + k * stride + (size_t)offs[e] * 4, (size_t)reqs[e]->n * 4,
+                                      hipMemcpyDeviceToHost, sl.stream);
+        if (r != hipSuccess) return r;
+        ++e;
+        continue;
+      }
+      size_t f = e;
+      while (f < reqs.size() && !reqs[f]->pinnedOut) ++f;
+      const size_t a = (size_t)offs[e] * 4, b = (size_t)offs[f] * 4;
+      hipError_t r = hipMemcpyAsync(stage + k * stride + a, dev + k * stride + a, b - a, hipMemcpyDeviceToHost, sl.stream);
+      if (r != hipSuccess) return r;
+      e = f;
+    }
+  }
+  return hipSuccess;
+}
+
 // Pack and launch the requests assigned to a slot. Returns CLUE_OK or an error code.
 int launch_slot(clue_server* s, Slot& sl) {
   clue_ctx* c = s->ctx;
@@ -132,12 +281,8 @@ int launch_slot(clue_server* s, Slot& sl) {
     sl.offs2.assign((size_t)E + 1, 0);
     for (int e = 0; e < E; ++e) sl.offs2[e + 1] = sl.offs2[e] + sl.reqs2[e]->n;
     const int64_t N = sl.offs2[E];
-    for (int e = 0; e < E; ++e)
-      for (int k = 0; k < kIn2; ++k)
-        std::memcpy(st + L.in2 + k * f2 + (size_t)sl.offs2[e] * 4, sl.reqs2[e]->in[k], (size_t)sl.reqs2[e]->n * 4);
     char* din = reinterpret_cast<char*>(sl.io2.base);
-    for (int k = 0; k < kIn2; ++k)
-      HIP_TRY(c, hipMemcpyAsync(din + k * f2, st + L.in2 + k * f2, (size_t)N * 4, hipMemcpyHostToDevice, sl.stream));
+    HIP_TRY(c, copy_in(s, sl, sl.reqs2, sl.offs2, kIn2, st + L.in2, din, f2));
     const int nb = [&] {
       int64_t b = 0;
       for (int e = 0; e < E; ++e) b += (sl.offs2[e + 1] - sl.offs2[e] + kSortBlock - 1) / kSortBlock;
@@ -161,8 +306,7 @@ int launch_slot(clue_server* s, Slot& sl) {
                 reinterpret_cast<int32_t*>(dout + 4 * f2)};
     clue2d_enqueue(b, io, s->p2, sl.stream);
     HIP_TRY(c, hipGetLastError());
-    for (int k = 0; k < kOut2; ++k)
-      HIP_TRY(c, hipMemcpyAsync(st + L.out2 + k * f2, dout + k * f2, (size_t)N * 4, hipMemcpyDeviceToHost, sl.stream));
+    HIP_TRY(c, copy_out(sl, sl.reqs2, sl.offs2, kOut2, st + L.out2, dout, f2));
   }
   // ---- 3D
   if (!sl.reqs3.empty()) {
@@ -170,12 +314,8 @@ int launch_slot(clue_server* s, Slot& sl) {
     sl.offs3.assign((size_t)E + 1, 0);
     for (int e = 0; e < E; ++e) sl.offs3[e + 1] = sl.offs3[e] + sl.reqs3[e]->n;
     const int64_t N = sl.offs3[E];
-    for (int e = 0; e < E; ++e)
-      for (int k = 0; k < kIn3; ++k)
-        std::memcpy(st + L.in3 + k * f3 + (size_t)sl.offs3[e] * 4, sl.reqs3[e]->in[k], (size_t)sl.reqs3[e]->n * 4);
     char* din = reinterpret_cast<char*>(sl.io3.base);
-    for (int k = 0; k < kIn3; ++k)
-      HIP_TRY(c, hipMemcpyAsync(din + k * f3, st + L.in3 + k * f3, (size_t)N * 4, hipMemcpyHostToDevice, sl.stream));
+    HIP_TRY(c, copy_in(s, sl, sl.reqs3, sl.offs3, kIn3, st + L.in3, din, f3));
     int64_t nb64 = 0;
     for (int e = 0; e < E; ++e) nb64 += (sl.offs3[e + 1] - sl.offs3[e] + kSortBlock - 1) / kSortBlock;
     const int nb = (int)nb64;
@@ -196,8 +336,7 @@ int launch_slot(clue_server* s, Slot& sl) {
                 F(dout, 0), F(dout, 1), I(dout, 2), I(dout, 3), I(dout, 4), I(dout, 5), I(dout, 6)};
     clue3d_enqueue(b, io, s->p3, sl.stream);
     HIP_TRY(c, hipGetLastError());
-    for (int k = 0; k < kOut3; ++k)
-      HIP_TRY(c, hipMemcpyAsync(st + L.out3 + k * f3, dout + k * f3, (size_t)N * 4, hipMemcpyDeviceToHost, sl.stream));
+    HIP_TRY(c, copy_out(sl, sl.reqs3, sl.offs3, kOut3, st + L.out3, dout, f3));
   }
   HIP_TRY(c, hipEventRecord(sl.done, sl.stream));
   return CLUE_OK;
@@ -209,13 +348,16 @@ void finish_slot(clue_server* s, Slot& sl, int status) {
   const size_t f2 = align256((size_t)s->cap2 * 4), f3 = align256((size_t)s->cap3 * 4);
   if (status == CLUE_OK) {
     for (size_t e = 0; e < sl.reqs2.size(); ++e)
-      for (int k = 0; k < kOut2; ++k)
-        std::memcpy(sl.reqs2[e]->out[k], sl.stage + L.out2 + k * f2 + (size_t)sl.offs2[e] * 4,
-                    (size_t)sl.reqs2[e]->n * 4);
+      if (!sl.reqs2[e]->pinnedOut)
+        for (int k = 0; k < kOut2; ++k)
+          s->copies->add(sl.reqs2[e]->out[k], sl.stage + L.out2 + k * f2 + (size_t)sl.offs2[e] * 4,
+                         (size_t)sl.reqs2[e]->n * 4);
     for (size_t e = 0; e < sl.reqs3.size(); ++e)
-      for (int k = 0; k < kOut3; ++k)
-        std::memcpy(sl.reqs3[e]->out[k], sl.stage + L.out3 + k * f3 + (size_t)sl.offs3[e] * 4,
-                    (size_t)sl.reqs3[e]->n * 4);
+      if (!sl.reqs3[e]->pinnedOut)
+        for (int k = 0; k < kOut3; ++k)
+          s->copies->add(sl.reqs3[e]->out[k], sl.stage + L.out3 + k * f3 + (size_t)sl.offs3[e] * 4,
+                         (size_t)sl.reqs3[e]->n * 4);
+    s->copies->run_all();
   }
   const auto now = Clock::now();
   {
@@ -321,6 +463,10 @@ int64_t submit(clue_server* s, int dim, int32_t n, const void* const* in, void* 
   r->n = n;
   for (int k = 0; k < nin; ++k) r->in[k] = in[k];
   for (int k = 0; k < nout; ++k) r->out[k] = out[k];
+  r->pinnedIn = true;
+  for (int k = 0; k < nin && n > 0; ++k) r->pinnedIn = r->pinnedIn && is_pinned(in[k]);
+  r->pinnedOut = true;
+  for (int k = 0; k < nout && n > 0; ++k) r->pinnedOut = r->pinnedOut && is_pinned(out[k]);
   r->tSubmit = Clock::now();
   int64_t id;
   {
@@ -390,6 +536,7 @@ clue_server* clue_server_create(clue_ctx* c, const clue2d_params* p2, const clue
     if (c->err.empty()) set_err(c, CLUE_ERR_HIP, "server resources could not be created");
     return nullptr;
   }
+  s->copies = std::make_unique<CopyPool>(7);
   clue_server* raw = s.release();
   raw->worker = std::thread(worker_loop, raw);
   return raw;
