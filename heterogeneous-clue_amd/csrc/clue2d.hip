@@ -67,25 +67,7 @@ struct Keys2D {
   }
 };
 
-struct EmitFirst {
-  uint32_t* kOut;
-  int* iOut;
-  __device__ __forceinline__ void operator()(int pos, int i, uint32_t k) const {
-    kOut[pos] = k;
-    iOut[pos] = i;
-  }
-};
-
-struct EmitMid {
-  const int* iIn;
-  uint32_t* kOut;
-  int* iOut;
-  __device__ __forceinline__ void operator()(int pos, int i, uint32_t k) const {
-    kOut[pos] = k;
-    iOut[pos] = iIn[i];
-  }
-};
-
+// Last pass: also gathers x, y, weight into tile order.
 struct EmitLast2D {
   const int* iIn;
   uint32_t* kOut;
@@ -94,11 +76,18 @@ struct EmitLast2D {
   const float* y;
   const float* w;
   float4* xyw;
-  __device__ __forceinline__ void operator()(int pos, int i, uint32_t k) const {
+  struct Payload {
+    int idx;
+    float x, y, w;
+  };
+  __device__ __forceinline__ Payload load(int i) const {
     const int idx = iIn[i];
+    return Payload{idx, x[idx], y[idx], w[idx]};
+  }
+  __device__ __forceinline__ void store(int pos, uint32_t k, const Payload& v) const {
     kOut[pos] = k;
-    iOut[pos] = idx;
-    xyw[pos] = make_float4(x[idx], y[idx], w[idx], 0.f);
+    iOut[pos] = v.idx;
+    xyw[pos] = make_float4(v.x, v.y, v.w, 0.f);
   }
 };
 

@@ -45,25 +45,7 @@ struct Keys3D {
   }
 };
 
-struct EmitFirst {
-  uint32_t* kOut;
-  int* iOut;
-  __device__ __forceinline__ void operator()(int pos, int i, uint32_t k) const {
-    kOut[pos] = k;
-    iOut[pos] = i;
-  }
-};
-
-struct EmitMid {
-  const int* iIn;
-  uint32_t* kOut;
-  int* iOut;
-  __device__ __forceinline__ void operator()(int pos, int i, uint32_t k) const {
-    kOut[pos] = k;
-    iOut[pos] = iIn[i];
-  }
-};
-
+// Last pass: also gathers r_over_absz, phi and energy into tile order.
 struct EmitLast3D {
   const int* iIn;
   uint32_t* kOut;
@@ -74,13 +56,20 @@ struct EmitLast3D {
   float* rozS;
   float* phiS;
   float* enS;
-  __device__ __forceinline__ void operator()(int pos, int i, uint32_t k) const {
+  struct Payload {
+    int idx;
+    float roz, phi, en;
+  };
+  __device__ __forceinline__ Payload load(int i) const {
     const int idx = iIn[i];
+    return Payload{idx, roz[idx], phi[idx], en[idx]};
+  }
+  __device__ __forceinline__ void store(int pos, uint32_t k, const Payload& v) const {
     kOut[pos] = k;
-    iOut[pos] = idx;
-    rozS[pos] = roz[idx];
-    phiS[pos] = phi[idx];
-    enS[pos] = en[idx];
+    iOut[pos] = v.idx;
+    rozS[pos] = v.roz;
+    phiS[pos] = v.phi;
+    enS[pos] = v.en;
   }
 };
 

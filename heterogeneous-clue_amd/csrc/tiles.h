@@ -67,42 +67,67 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(const BlockDesc* __rest
   __shared__ int s_h[kRadix];
   const BlockDesc bd = blocks[blockIdx.x];
   if (threadIdx.x < kRadix) s_h[threadIdx.x] = 0;
-  __syncthreads();
+  // all of the thread's keys in flight at once
+  int d[kSortItems];
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     const int i = bd.begin + r * kThreads + (int)threadIdx.x;
-    const bool valid = i < bd.end;
-    const int d = valid ? D::get(src(i)) : kRadix - 1;
-    const uint64_t m = match_digit7(d);
+    d[r] = i < bd.end ? D::get(src(i)) : kRadix - 1;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const bool valid = bd.begin + r * kThreads + (int)threadIdx.x < bd.end;
+    const uint64_t m = match_digit7(d[r]);
     const int before = __popcll(m & ((1ull << lane_id()) - 1ull));
-    if (valid && before == 0) atomicAdd(&s_h[d], (int)__popcll(m));
+    if (valid && before == 0) atomicAdd(&s_h[d[r]], (int)__popcll(m));
   }
   __syncthreads();
   if (threadIdx.x < kRadix) hist[bd.histBase + (int)threadIdx.x * bd.nbEvent + bd.bLocal] = s_h[threadIdx.x];
 }
 
-// Emit(pos, i, key): write the item read from input position i at sorted position pos.
+// Emit::load(i) reads what item i carries (issued for all of a thread's items before ranking);
+// Emit::store(pos, key, payload) writes it at its sorted position. The block first ranks its items
+// stably by digit (wave ballot match + per-wave counts, round by round in input order), places
+// them in LDS in (digit, rank) order, and then writes each digit's run to its global position with
+// consecutive threads on consecutive positions.
 template <class Src, class D, class Emit>
 __global__ __launch_bounds__(kThreads) void k_radix_scatter(const BlockDesc* __restrict__ blocks, Src src,
                                                             const int* __restrict__ histScanned, Emit emit) {
+  using Payload = typename Emit::Payload;
   __shared__ int s_base[kRadix];
   __shared__ int s_run[kRadix];
+  __shared__ int s_start[kRadix];
   __shared__ int s_cnt[kWaves][kRadix];
   __shared__ int s_off[kWaves][kRadix];
+  __shared__ uint32_t s_key[kSortBlock];
+  __shared__ Payload s_pay[kSortBlock];
   const BlockDesc bd = blocks[blockIdx.x];
   const int tid = (int)threadIdx.x, lane = lane_id(), wave = tid / kWave;
+  const int nItems = bd.end - bd.begin;
+  uint32_t key[kSortItems];
+  Payload pay[kSortItems];
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int i = bd.begin + r * kThreads + tid;
+    key[r] = i < bd.end ? src(i) : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int i = bd.begin + r * kThreads + tid;
+    pay[r] = emit.load(i < bd.end ? i : bd.begin);
+  }
   if (tid < kRadix) {
     s_base[tid] = histScanned[bd.histBase + tid * bd.nbEvent + bd.bLocal];
     s_run[tid] = 0;
   }
   for (int k = lane; k < kRadix; k += kWave) s_cnt[wave][k] = 0;
   __syncthreads();
-#pragma unroll 1
+  int rank[kSortItems];  // position among the block's items of the same digit
+#pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    const int i = bd.begin + r * kThreads + tid;
-    const bool valid = i < bd.end;
-    const uint32_t key = valid ? src(i) : 0u;
-    const int d = valid ? D::get(key) : kRadix - 1;
+    const bool valid = bd.begin + r * kThreads + tid < bd.end;
+    const int d = valid ? D::get(key[r]) : kRadix - 1;
     const uint64_t m = match_digit7(d);
     const int before = __popcll(m & ((1ull << lane) - 1ull));
     if (valid && before == 0) s_cnt[wave][d] = (int)__popcll(m);
@@ -117,10 +142,57 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const BlockDesc* __r
       s_run[tid] = run;
     }
     __syncthreads();
-    if (valid) emit(s_base[d] + s_off[wave][d] + before, i, key);
+    rank[r] = s_off[wave][d] + before;
     for (int k = lane; k < kRadix; k += kWave) s_cnt[wave][k] = 0;
   }
+  // start of each digit's run within the block (exclusive scan of the block's digit counts)
+  if (tid < kWave) {
+    int v0 = s_run[2 * tid], v1 = s_run[2 * tid + 1];
+    const int inc = wave_inclusive_sum(v0 + v1);
+    s_start[2 * tid] = inc - v0 - v1;
+    s_start[2 * tid + 1] = inc - v1;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    if (bd.begin + r * kThreads + tid < bd.end) {
+      const int pos = s_start[D::get(key[r])] + rank[r];
+      s_key[pos] = key[r];
+      s_pay[pos] = pay[r];
+    }
+  }
+  __syncthreads();
+  for (int j = tid; j < nItems; j += kThreads) {
+    const uint32_t k = s_key[j];
+    const int d = D::get(k);
+    emit.store(s_base[d] + (j - s_start[d]), k, s_pay[j]);
+  }
 }
+
+// First pass: the item is its input position.
+struct EmitFirst {
+  uint32_t* kOut;
+  int* iOut;
+  using Payload = int;
+  __device__ __forceinline__ Payload load(int i) const { return i; }
+  __device__ __forceinline__ void store(int pos, uint32_t k, Payload v) const {
+    kOut[pos] = k;
+    iOut[pos] = v;
+  }
+};
+
+// Middle passes: the item carries the point index it was given by the previous pass.
+struct EmitMid {
+  const int* iIn;
+  uint32_t* kOut;
+  int* iOut;
+  using Payload = int;
+  __device__ __forceinline__ Payload load(int i) const { return iIn[i]; }
+  __device__ __forceinline__ void store(int pos, uint32_t k, Payload v) const {
+    kOut[pos] = k;
+    iOut[pos] = v;
+  }
+};
 
 // Dense bin table: table[e * stride + b] = first sorted position of event e whose key is >= b,
 // for b in [0, nKeys]. A first pass over the sorted keys records where each tile of kTableTile
