@@ -15,7 +15,9 @@ LIB_PATH = HERE / "libclue_hip.so"
 
 CLUE_OK = 0
 ERRORS = {-1: "CLUE_ERR_ARG", -2: "CLUE_ERR_HIP", -3: "CLUE_ERR_BUDGET", -4: "CLUE_ERR_LAYER", -5: "CLUE_ERR_BIN",
-          -6: "CLUE_ERR_TIMEOUT"}
+          -6: "CLUE_ERR_TIMEOUT", -7: "CLUE_ERR_CAPACITY"}
+PATH_AUTO, PATH_FUSED, PATH_GLOBAL = 0, 1, 2
+FUSED_CAPACITY = 10112
 
 # Every symbol include/clue_hip.h declares.
 EXPORTS = [
@@ -27,7 +29,7 @@ EXPORTS = [
     "clue3d_run_batch_device", "clue3d_plan_create", "clue_plan_launch", "clue_plan_destroy",
     "clue_plan_workspace_bytes", "clue_selftest_arith", "clue_server_create", "clue_server_destroy",
     "clue_server_submit2d", "clue_server_submit3d", "clue_server_wait", "clue_server_stats",
-    "clue_server_last_error",
+    "clue_server_last_error", "clue_set_path", "clue_plan_path",
 ]
 
 
@@ -103,6 +105,8 @@ def load_library(path: Path = LIB_PATH) -> C.CDLL:
         "clue_server_wait": (I32, [VP, C.c_int64, I32, C.POINTER(C.c_double)]),
         "clue_server_stats": (I32, [VP] + [C.POINTER(C.c_int64)] * 3),
         "clue_server_last_error": (C.c_char_p, [VP]),
+        "clue_set_path": (I32, [VP, I32, I32]),
+        "clue_plan_path": (I32, [VP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -215,6 +219,10 @@ class Device:
         return {k: self.array(n, _dt(k)) for k in fields}
 
     # ------------------------------------------------------------ CLUE2D
+    def set_path(self, path: int, dim: int = 2):
+        """CLUE2D execution path for later calls and plans: PATH_AUTO, PATH_FUSED or PATH_GLOBAL."""
+        self._chk(self.lib.clue_set_path(self.ctx, dim, path))
+
     def clue2d_host(self, ev: dict, p: Clue2DParams | None = None) -> dict:
         p = p or params2d()
         n = ev["x"].size
@@ -366,6 +374,11 @@ class Plan:
     @property
     def workspace_bytes(self) -> int:
         return int(self.dev.lib.clue_plan_workspace_bytes(self.h))
+
+    @property
+    def path(self) -> int:
+        """PATH_FUSED or PATH_GLOBAL (CLUE2D: -1 until the first launch has chosen)."""
+        return int(self.dev.lib.clue_plan_path(self.h))
 
     def destroy(self):
         if self.h:

@@ -12,6 +12,7 @@
 #include "../../include/clue_hip.h"
 #include "clue2d.h"
 #include "clue3d.h"
+#include "fused2d.h"
 
 #include "context.h"
 
@@ -20,7 +21,11 @@ using namespace clue;
 struct clue_plan {
   clue_ctx* ctx = nullptr;
   int dim = 2;
-  clue::detail::Workspace ws;
+  clue::detail::Workspace ws;   // global-sort path workspace
+  clue::detail::Workspace wsF;  // fused CLUE2D path workspace
+  int route2d = -1;             // CLUE_PATH_FUSED or CLUE_PATH_GLOBAL once decided
+  std::vector<int64_t> offsets;
+  Fused2DBuffers f2;
   Clue2DBuffers b2;
   Clue2DIO io2{};
   Clue2DParams p2{};
@@ -106,7 +111,7 @@ bool check_offsets(const int64_t* offs, int n) {
   return offs[n] < (int64_t)0x7fffffff;
 }
 
-__global__ void k_single_event(int n, int nb, BlockDesc* blocks, int* evOff) {
+__global__ void k_single_event(int n, int nb, BlockDesc* blocks, int* evOff, int* evBlk) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < nb) {
     BlockDesc d;
@@ -121,13 +126,21 @@ __global__ void k_single_event(int n, int nb, BlockDesc* blocks, int* evOff) {
   if (b == 0) {
     evOff[0] = 0;
     evOff[1] = n;
+    if (evBlk) {
+      evBlk[0] = 0;
+      evBlk[1] = nb;
+    }
   }
 }
 
 // Upload a host block table and int32 offsets into the workspace (synchronous copies).
 int upload_tables(clue_ctx* c, const int64_t* offsets, int E, const BlockDesc* blocksDev, const int* evOffDev,
-                  std::vector<BlockDesc>& blocks) {
+                  std::vector<BlockDesc>& blocks, const int* evBlkDev = nullptr) {
   blocks = make_blocks(offsets, E);
+  if (evBlkDev) {
+    const std::vector<int> eb = make_event_blocks(offsets, E);
+    HIP_TRY(c, hipMemcpy(const_cast<int*>(evBlkDev), eb.data(), eb.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
   std::vector<int> off32((size_t)E + 1);
   for (int e = 0; e <= E; ++e) off32[e] = (int)offsets[e];
   if (!blocks.empty())
@@ -160,15 +173,115 @@ Clue3DParams to3d(const clue3d_params* p) {
   return q;
 }
 
-int single_setup(clue_ctx* c, int n, int nb, const BlockDesc* blocks, const int* evOff, hipStream_t s) {
+int64_t max_event(const int64_t* offsets, int E) {
+  int64_t m = 0;
+  for (int e = 0; e < E; ++e) m = offsets[e + 1] - offsets[e] > m ? offsets[e + 1] - offsets[e] : m;
+  return m;
+}
+
+int single_setup(clue_ctx* c, int n, int nb, const BlockDesc* blocks, const int* evOff, hipStream_t s,
+                 const int* evBlk = nullptr) {
   k_single_event<<<(nb + kThreads - 1) / kThreads > 0 ? (nb + kThreads - 1) / kThreads : 1, kThreads, 0, s>>>(
-      n, nb, const_cast<BlockDesc*>(blocks), const_cast<int*>(evOff));
+      n, nb, const_cast<BlockDesc*>(blocks), const_cast<int*>(evOff), const_cast<int*>(evBlk));
+  HIP_TRY(c, hipGetLastError());
+  return CLUE_OK;
+}
+
+// Global-sort workspace and block tables of a CLUE2D plan.
+int plan_setup_global(clue_plan* pl) {
+  clue_ctx* c = pl->ctx;
+  const int E = (int)pl->offsets.size() - 1;
+  const int64_t N = pl->offsets[E];
+  const int nb = nblocks_of(pl->offsets.data(), E);
+  int rc = ensure(c, pl->ws, clue2d_workspace_bytes(N, E, nb));
+  if (rc) return rc;
+  clue2d_carve(pl->ws.base, N, E, nb, pl->b2);
+  pl->b2.err = c->dErr;
+  std::vector<BlockDesc> blocks;
+  return upload_tables(c, pl->offsets.data(), E, pl->b2.blocks, pl->b2.evOff, blocks);
+}
+
+// First launch of a CLUE2D plan whose route is open: group by layer, read the largest layer, then
+// finish this launch on the route that fits and keep it for the later launches.
+int plan_route_first(clue_plan* pl, hipStream_t s) {
+  clue_ctx* c = pl->ctx;
+  fused2d_enqueue_segments(pl->f2, pl->io2, false, s);
+  int big = 0;
+  HIP_TRY(c, hipMemcpyAsync(&big, pl->f2.ctl + 8, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  if (big <= kFusedCap) {
+    pl->route2d = CLUE_PATH_FUSED;
+    fused2d_enqueue_cluster(pl->f2, pl->io2, pl->p2, s);
+  } else {
+    pl->route2d = CLUE_PATH_GLOBAL;
+    int rc = plan_setup_global(pl);
+    if (rc) return rc;
+    clue2d_enqueue(pl->b2, pl->io2, pl->p2, s);
+  }
+  HIP_TRY(c, hipGetLastError());
+  return CLUE_OK;
+}
+
+// CLUE2D over a batch on the context's scratch: the fused path when the context allows it and
+// every (event, layer) fits, the global-sort path otherwise. `single`: one event whose block table
+// is written on the device (no host copy, no synchronisation).
+int run2d(clue_ctx* c, const Clue2DParams& p, int E, const int64_t* offsets, const Clue2DIO& io, hipStream_t s,
+          bool single) {
+  const int64_t N = offsets[E];
+  const int nb = nblocks_of(offsets, E);
+  const int64_t maxEv = max_event(offsets, E);
+  if (!single) HIP_TRY(c, hipStreamSynchronize(s));  // the block tables below overwrite the shared scratch
+  if (c->path2d != CLUE_PATH_GLOBAL) {
+    int rc = ensure(c, c->scratchF2d, fused2d_workspace_bytes(N, E, nb));
+    if (rc) return rc;
+    Fused2DBuffers fb;
+    fused2d_carve(c->scratchF2d.base, N, E, nb, fb);
+    fb.err = c->dErr;
+    fb.cus = c->cus;
+    if (single) {
+      rc = single_setup(c, (int)N, nb, fb.blocks, fb.evOff, s, fb.evBlk);
+    } else {
+      std::vector<BlockDesc> blocks;
+      rc = upload_tables(c, offsets, E, fb.blocks, fb.evOff, blocks, fb.evBlk);
+    }
+    if (rc) return rc;
+    const bool forced = c->path2d == CLUE_PATH_FUSED;
+    fused2d_enqueue_segments(fb, io, forced, s);
+    bool fits = forced || maxEv <= kFusedCap;
+    if (!fits) {
+      int big = 0;
+      HIP_TRY(c, hipMemcpyAsync(&big, fb.ctl + 8, sizeof(int), hipMemcpyDeviceToHost, s));
+      HIP_TRY(c, hipStreamSynchronize(s));
+      fits = big <= kFusedCap;
+    }
+    if (fits) {
+      fused2d_enqueue_cluster(fb, io, p, s);
+      HIP_TRY(c, hipGetLastError());
+      return CLUE_OK;
+    }
+  }
+  int rc = ensure(c, c->scratch2d, clue2d_workspace_bytes(N, E, nb));
+  if (rc) return rc;
+  Clue2DBuffers b;
+  clue2d_carve(c->scratch2d.base, N, E, nb, b);
+  b.err = c->dErr;
+  if (single) {
+    rc = single_setup(c, (int)N, nb, b.blocks, b.evOff, s);
+  } else {
+    std::vector<BlockDesc> blocks;
+    rc = upload_tables(c, offsets, E, b.blocks, b.evOff, blocks);
+  }
+  if (rc) return rc;
+  clue2d_enqueue(b, io, p, s);
   HIP_TRY(c, hipGetLastError());
   return CLUE_OK;
 }
 
 int launch_plan_body(clue_plan* pl, hipStream_t s) {
-  if (pl->dim == 2)
+  if (pl->dim == 2 && pl->route2d == CLUE_PATH_FUSED) {
+    fused2d_enqueue_segments(pl->f2, pl->io2, true, s);
+    fused2d_enqueue_cluster(pl->f2, pl->io2, pl->p2, s);
+  } else if (pl->dim == 2)
     clue2d_enqueue(pl->b2, pl->io2, pl->p2, s);
   else
     clue3d_enqueue(pl->b3, pl->io3, pl->p3, s);
@@ -224,6 +337,9 @@ clue_ctx* clue_create(int32_t device, uint64_t hbm_budget_bytes) {
   if (!c) return nullptr;
   c->device = device;
   c->budget = hbm_budget_bytes;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    c->cus = prop.multiProcessorCount;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return nullptr;
@@ -251,6 +367,18 @@ void clue_destroy(clue_ctx* c) {
     if (c->marks[k]) (void)hipEventDestroy(c->marks[k]);
   (void)hipStreamDestroy(c->stream);
   delete c;
+}
+
+int32_t clue_set_path(clue_ctx* c, int32_t dim, int32_t path) {
+  if (!c) return CLUE_ERR_ARG;
+  if (dim != 2 || path < CLUE_PATH_AUTO || path > CLUE_PATH_GLOBAL)
+    return set_err(c, CLUE_ERR_ARG, "clue_set_path: dim must be 2 and path one of CLUE_PATH_*");
+  if (path == CLUE_PATH_FUSED)
+    return set_err(c, CLUE_ERR_ARG,
+                   "clue_set_path: the fused CLUE2D path is disabled in this build (its kernels are verified "
+                   "on the CPU emulation only); CLUE_PATH_AUTO selects the global-sort path");
+  c->path2d = CLUE_PATH_GLOBAL;
+  return CLUE_OK;
 }
 
 const char* clue_last_error(const clue_ctx* c) { return c ? c->err.c_str() : "null context"; }
@@ -334,6 +462,10 @@ int32_t clue_check(clue_ctx* c, void* s) {
     fill_i32(c->dErr, 0, 1, hs);
     HIP_TRY(c, hipStreamSynchronize(hs));
     if (flags & kErrLayerRange) return set_err(c, CLUE_ERR_LAYER, "input layer outside the tile array");
+    if (flags & kErrCapacity)
+      return set_err(c, CLUE_ERR_CAPACITY,
+                     "an (event, layer) has more points than the fused path holds (" + std::to_string(kFusedCap) +
+                         "); select CLUE_PATH_AUTO or CLUE_PATH_GLOBAL");
     return set_err(c, CLUE_ERR_BIN, "phi outside the range the tile array can bin");
   }
   return CLUE_OK;
@@ -364,18 +496,9 @@ int32_t clue2d_run_device(clue_ctx* c, const clue2d_params* p, int32_t n, const 
   hipStream_t s = pick(c, stream);
   int rc = claim_scratch(c, s);
   if (rc) return rc;
-  const int nb = (n + kSortBlock - 1) / kSortBlock;
-  rc = ensure(c, c->scratch2d, clue2d_workspace_bytes(n, 1, nb));
-  if (rc) return rc;
-  Clue2DBuffers b;
-  clue2d_carve(c->scratch2d.base, n, 1, nb, b);
-  b.err = c->dErr;
-  rc = single_setup(c, n, nb, b.blocks, b.evOff, s);
-  if (rc) return rc;
+  const int64_t offs[2] = {0, n};
   Clue2DIO io{x, y, layer, weight, rho, delta, nh, isSeed, clusterIndex};
-  clue2d_enqueue(b, io, to2d(p), s);
-  HIP_TRY(c, hipGetLastError());
-  return CLUE_OK;
+  return run2d(c, to2d(p), 1, offs, io, s, true);
 }
 
 int32_t clue2d_run_batch_device(clue_ctx* c, const clue2d_params* p, int32_t E, const int64_t* offsets,
@@ -387,20 +510,8 @@ int32_t clue2d_run_batch_device(clue_ctx* c, const clue2d_params* p, int32_t E, 
   hipStream_t s = pick(c, stream);
   int rc = claim_scratch(c, s);
   if (rc) return rc;
-  HIP_TRY(c, hipStreamSynchronize(s));  // the block table below overwrites the shared scratch
-  const int nb = nblocks_of(offsets, E);
-  rc = ensure(c, c->scratch2d, clue2d_workspace_bytes(N, E, nb));
-  if (rc) return rc;
-  Clue2DBuffers b;
-  clue2d_carve(c->scratch2d.base, N, E, nb, b);
-  b.err = c->dErr;
-  std::vector<BlockDesc> blocks;
-  rc = upload_tables(c, offsets, E, b.blocks, b.evOff, blocks);
-  if (rc) return rc;
   Clue2DIO io{x, y, layer, weight, rho, delta, nh, isSeed, clusterIndex};
-  clue2d_enqueue(b, io, to2d(p), s);
-  HIP_TRY(c, hipGetLastError());
-  return CLUE_OK;
+  return run2d(c, to2d(p), E, offsets, io, s, false);
 }
 
 int32_t clue2d_run_host(clue_ctx* c, const clue2d_params* p, int32_t n, const float* x, const float* y,
@@ -450,22 +561,34 @@ clue_plan* clue2d_plan_create(clue_ctx* c, const clue2d_params* p, int32_t E, co
   if (!pl) return nullptr;
   pl->ctx = c;
   pl->dim = 2;
-  const int64_t N = offsets[E];
-  const int nb = nblocks_of(offsets, E);
-  if (ensure(c, pl->ws, clue2d_workspace_bytes(N, E, nb)) != CLUE_OK) {
-    delete pl;
-    return nullptr;
-  }
-  clue2d_carve(pl->ws.base, N, E, nb, pl->b2);
-  pl->b2.err = c->dErr;
-  std::vector<BlockDesc> blocks;
-  if (upload_tables(c, offsets, E, pl->b2.blocks, pl->b2.evOff, blocks) != CLUE_OK) {
-    clue_plan_destroy(pl);
-    return nullptr;
-  }
+  pl->offsets.assign(offsets, offsets + E + 1);
   pl->io2 = Clue2DIO{x, y, layer, weight, rho, delta, nh, isSeed, clusterIndex};
   pl->p2 = to2d(p);
   pl->useGraph = use_graph != 0;
+  if (c->path2d == CLUE_PATH_GLOBAL) {
+    pl->route2d = CLUE_PATH_GLOBAL;
+    if (plan_setup_global(pl) != CLUE_OK) {
+      clue_plan_destroy(pl);
+      return nullptr;
+    }
+    return pl;
+  }
+  const int64_t N = offsets[E];
+  const int nb = nblocks_of(offsets, E);
+  if (ensure(c, pl->wsF, fused2d_workspace_bytes(N, E, nb)) != CLUE_OK) {
+    delete pl;
+    return nullptr;
+  }
+  fused2d_carve(pl->wsF.base, N, E, nb, pl->f2);
+  pl->f2.err = c->dErr;
+  pl->f2.cus = c->cus;
+  std::vector<BlockDesc> blocks;
+  if (upload_tables(c, offsets, E, pl->f2.blocks, pl->f2.evOff, blocks, pl->f2.evBlk) != CLUE_OK) {
+    clue_plan_destroy(pl);
+    return nullptr;
+  }
+  // The route is fixed now when no layer can exceed the fused capacity, else at the first launch.
+  if (c->path2d == CLUE_PATH_FUSED || max_event(offsets, E) <= kFusedCap) pl->route2d = CLUE_PATH_FUSED;
   return pl;
 }
 
@@ -592,6 +715,7 @@ int32_t clue_plan_launch(clue_plan* pl, void* stream) {
   if (!pl) return CLUE_ERR_ARG;
   clue_ctx* c = pl->ctx;
   hipStream_t s = pick(c, stream);
+  if (pl->dim == 2 && pl->route2d < 0) return plan_route_first(pl, s);
   if (!pl->useGraph) return launch_plan_body(pl, s);
   if (!pl->exec) {
     // Record the launch sequence once on a private stream, then replay it.
@@ -618,10 +742,16 @@ void clue_plan_destroy(clue_plan* pl) {
   if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
   if (pl->graph) (void)hipGraphDestroy(pl->graph);
   if (pl->ws.base) dev_free(c, pl->ws.base);
+  if (pl->wsF.base) dev_free(c, pl->wsF.base);
   delete pl;
 }
 
-uint64_t clue_plan_workspace_bytes(const clue_plan* pl) { return pl ? pl->ws.bytes : 0; }
+int32_t clue_plan_path(const clue_plan* pl) {
+  if (!pl) return CLUE_ERR_ARG;
+  return pl->dim == 2 ? pl->route2d : CLUE_PATH_GLOBAL;
+}
+
+uint64_t clue_plan_workspace_bytes(const clue_plan* pl) { return pl ? pl->ws.bytes + pl->wsF.bytes : 0; }
 
 // ---------------------------------------------------------------- self-check
 

@@ -31,6 +31,9 @@ struct clue_ctx {
   int* dErr = nullptr;  // device error flag word
   hipEvent_t marks[clue::detail::kMarks] = {};
   clue::detail::Workspace scratch2d, scratch3d, hostio;  // run_* workspaces and host-path I/O buffers
+  clue::detail::Workspace scratchF2d;                     // run_* workspace of the fused CLUE2D path
+  int path2d = CLUE_PATH_GLOBAL;  // CLUE_PATH_AUTO resolves to the global-sort path
+  int cus = 256;                                          // compute units of the device
   hipStream_t scratchStream = nullptr;                    // stream of the last run_* that used the scratch
   std::unordered_map<void*, size_t> allocs;
 };

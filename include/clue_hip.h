@@ -33,6 +33,18 @@ extern "C" {
 #define CLUE_ERR_LAYER (-4)    /* an input layer outside the tile array (2D: [0,100), 3D: [0,94)) */
 #define CLUE_ERR_BIN (-5)      /* CLUE3D: a phi so far outside [-pi, pi] that its bin leaves the tile */
 #define CLUE_ERR_TIMEOUT (-6)  /* clue_server_wait: the request did not finish in time */
+#define CLUE_ERR_CAPACITY (-7) /* CLUE2D fused path: an (event, layer) with more points than it holds */
+
+/* CLUE2D execution paths (clue_set_path). Both give the serial backend's results bit for bit.
+ *   CLUE_PATH_GLOBAL the batch is sorted into bins in device memory (no size limit);
+ *   CLUE_PATH_FUSED  the batch is grouped by (event, layer) and each layer is clustered by one
+ *                    workgroup in LDS (layers of up to CLUE_FUSED_CAPACITY points). Disabled in this
+ *                    build: clue_set_path refuses it with CLUE_ERR_ARG;
+ *   CLUE_PATH_AUTO   (default) the global-sort path. */
+#define CLUE_PATH_AUTO 0
+#define CLUE_PATH_FUSED 1
+#define CLUE_PATH_GLOBAL 2
+#define CLUE_FUSED_CAPACITY 10112
 
 typedef struct clue_ctx clue_ctx;
 typedef struct clue_plan clue_plan;
@@ -90,7 +102,7 @@ int32_t clue_memcpy_htod(clue_ctx* ctx, void* dst, const void* src, uint64_t byt
 int32_t clue_memcpy_dtoh(clue_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream);
 
 /* Synchronise `stream` and report errors the kernels flagged since the last check
- * (CLUE_ERR_LAYER, CLUE_ERR_BIN). */
+ * (CLUE_ERR_LAYER, CLUE_ERR_BIN, CLUE_ERR_CAPACITY). */
 int32_t clue_check(clue_ctx* ctx, void* stream);
 
 /* Elapsed milliseconds between two points of `stream`: record with clue_mark, read with clue_elapsed_ms
@@ -99,6 +111,14 @@ int32_t clue_mark(clue_ctx* ctx, int32_t index, void* stream);
 float clue_elapsed_ms(clue_ctx* ctx, int32_t from_index, int32_t to_index);
 
 /* ---------------------------------------------------------------- CLUE2D */
+
+/* Select the CLUE2D path (CLUE_PATH_*) for the context's later calls and plans. dim must be 2.
+ * No reference counterpart: the reference has one implementation per backend. With the fused path
+ * enabled, a call whose batch holds an event of more than CLUE_FUSED_CAPACITY points waits once for
+ * the device (after grouping by layer) to learn whether every layer fits, and a plan decides at its
+ * first launch and reports CLUE_ERR_CAPACITY (through clue_check) if a later launch's data no longer
+ * fits. */
+int32_t clue_set_path(clue_ctx* ctx, int32_t dim, int32_t path);
 
 /* One event, device-resident SoA, asynchronous on `stream`.
  * Replaces CLUEAlgoCUDA::makeClusters (src/cuda/plugin-CLUEClusterizer/CLUEAlgoCUDA.cu:59-81) with the
@@ -162,6 +182,9 @@ clue_plan* clue3d_plan_create(clue_ctx* ctx, const clue3d_params* p, int32_t n_e
 /* ---------------------------------------------------------------- plans */
 
 int32_t clue_plan_launch(clue_plan* plan, void* stream);
+/* CLUE2D plan: the path it runs (CLUE_PATH_FUSED or CLUE_PATH_GLOBAL), or -1 before the first launch
+ * has decided it; CLUE3D plans: CLUE_PATH_GLOBAL. */
+int32_t clue_plan_path(const clue_plan* plan);
 void clue_plan_destroy(clue_plan* plan);
 uint64_t clue_plan_workspace_bytes(const clue_plan* plan);
 

@@ -1,0 +1,149 @@
+"""CLUE2D execution paths: the global-sort path on every golden case and batch shape, the fused
+segment path's routing and capacity rules.
+
+The fused path is disabled in the library (clue_set_path refuses it): its first device run faulted
+the GPU and the cause is not yet found, so every test that would run it on the device is skipped.
+Its kernels are checked on the CPU emulation instead (test_cpu_emulation.py)."""
+import numpy as np
+import pytest
+
+import clue_hip
+import helpers
+import synth
+
+pytestmark = pytest.mark.gpu
+
+PATHS = {"fused": clue_hip.PATH_FUSED, "global": clue_hip.PATH_GLOBAL, "auto": clue_hip.PATH_AUTO}
+FUSED_OFF = "fused CLUE2D path disabled on the device (first device run faulted; cause not found yet)"
+fused_param = pytest.param("fused", marks=pytest.mark.skip(reason=FUSED_OFF))
+
+
+@pytest.fixture
+def on_path(dev):
+    """Select a path for one test and restore the default afterwards."""
+    def select(name):
+        dev.set_path(PATHS[name])
+    yield select
+    dev.set_path(clue_hip.PATH_AUTO)
+
+
+def run_batch(dev, packed, offs, p=None):
+    N = int(offs[-1])
+    din = dev.upload_soa(packed, clue_hip.FIELDS_2D_IN)
+    dout = dev.alloc_soa(N, clue_hip.FIELDS_2D_OUT)
+    dev.clue2d_batch(din, dout, offs, p)
+    dev.check()
+    out = dev.download_soa(dout)
+    for a in list(din.values()) + list(dout.values()):
+        a.free()
+    return out
+
+
+@pytest.mark.parametrize("path", [fused_param, "global"])
+@pytest.mark.parametrize("name", helpers.golden_names("clue2d"))
+def test_golden_each_path(dev, on_path, path, name):
+    ins, want, extra = helpers.load_case("clue2d", name)
+    dc, rhoc, odf = (float(v) for v in extra["params"])
+    print(f"case clue2d {path} {name} n={ins['x'].size}", flush=True)
+    on_path(path)
+    got = dev.clue2d_host(ins, clue_hip.params2d(dc, rhoc, odf))
+    helpers.assert_bit_exact(got, want, helpers.OUT2D, f"{path} {name}")
+
+
+def layered_event(counts, seed, dense=False):
+    """An event with counts[l] points on layer l (blob model; dense: a few tight clumps)."""
+    rng = np.random.Generator(np.random.PCG64([seed, 77]))
+    xs, ys, ls = [], [], []
+    for l, n in enumerate(counts):
+        if n == 0:
+            continue
+        if dense:
+            c = rng.uniform(-20, 20, (4, 2))
+            pick = rng.integers(0, 4, n)
+            pts = c[pick] + rng.normal(0, 0.4, (n, 2))
+        else:
+            pts = rng.uniform(-250, 250, (n, 2))
+            nb = n * 9 // 10
+            centres = rng.uniform(-250, 250, (max(1, nb // 20), 2))
+            pts[:nb] = centres[rng.integers(0, centres.shape[0], nb)] + rng.normal(0, 1.0, (nb, 2))
+        xs.append(pts[:, 0]); ys.append(pts[:, 1]); ls.append(np.full(n, l))
+    x = np.concatenate(xs).astype(np.float32)
+    y = np.concatenate(ys).astype(np.float32)
+    layer = np.concatenate(ls).astype(np.int32)
+    perm = rng.permutation(x.size)  # layers interleaved in index order
+    w = rng.exponential(1.0, x.size).astype(np.float32)
+    return {"x": x[perm], "y": y[perm], "layer": layer[perm], "weight": w}
+
+
+@pytest.mark.parametrize("path", [fused_param, "global"])
+def test_segment_class_boundaries(dev, oracle_lib, on_path, path):
+    """Layers of exactly the class sizes and one past them, in one interleaved event."""
+    counts = [1, 2, 1023, 1024, 1025, 4095, 4096, 4097, 10111, 10112, 0, 7]
+    ev = layered_event(counts, 1)
+    print(f"case clue2d class boundaries {path} n={ev['x'].size}", flush=True)
+    want = oracle_lib.clue2d(ev["x"], ev["y"], ev["layer"], ev["weight"])
+    on_path(path)
+    helpers.assert_bit_exact(dev.clue2d_host(ev), want, helpers.OUT2D, f"boundaries {path}")
+
+
+@pytest.mark.skip(reason=FUSED_OFF)
+def test_dense_full_layer(dev, oracle_lib, on_path):
+    """A full-capacity layer of tight clumps: long bins, many equal-distance candidates."""
+    ev = layered_event([0, 0, 10112, 300], 2, dense=True)
+    print(f"case clue2d dense layer n={ev['x'].size}", flush=True)
+    want = oracle_lib.clue2d(ev["x"], ev["y"], ev["layer"], ev["weight"])
+    on_path("fused")
+    got = dev.clue2d_host(ev)
+    helpers.assert_bit_exact(got, want, helpers.OUT2D, "dense fused")
+    helpers.check_invariants_2d(ev, got, 3.0, 10.0, 2.0)
+
+
+def test_layer_above_capacity(dev, oracle_lib, on_path):
+    """A layer of capacity + 1 points: the default and the global path cluster it bit-exactly."""
+    ev = layered_event([5, clue_hip.FUSED_CAPACITY + 1, 40], 3)
+    print(f"case clue2d over capacity n={ev['x'].size}", flush=True)
+    want = oracle_lib.clue2d(ev["x"], ev["y"], ev["layer"], ev["weight"])
+    for path in ("auto", "global"):
+        on_path(path)
+        helpers.assert_bit_exact(dev.clue2d_host(ev), want, helpers.OUT2D, f"over capacity {path}")
+
+
+def test_mixed_batch_all_paths_agree(dev, oracle_lib, on_path):
+    sizes = [1000, 0, 1, 23_456, 150_000, 3_333, 70_000, 2_048, 2_049, 120, 1_000_000]
+    packed, offs = synth.pack([synth.clue2d_event(n, 900 + i, shuffle=(i % 2 == 0)) for i, n in enumerate(sizes)])
+    print(f"case clue2d mixed batch all paths n={int(offs[-1])}", flush=True)
+    want = oracle_lib.clue2d_batch(packed, offs)
+    for path in ("auto", "global"):
+        on_path(path)
+        helpers.assert_bit_exact(run_batch(dev, packed, offs), want, helpers.OUT2D, f"mixed {path}")
+
+
+def test_plan_path_and_replays(dev, oracle_lib):
+    """Plans run the global-sort path from creation on; graph replays stay bit-exact."""
+    big = layered_event([clue_hip.FUSED_CAPACITY + 500, 10], 4)
+    packed, offs = synth.pack([synth.clue2d_event(3000, 5), big])
+    N = int(offs[-1])
+    print(f"case clue2d plan path n={N}", flush=True)
+    want = oracle_lib.clue2d_batch(packed, offs)
+    din = dev.upload_soa(packed, clue_hip.FIELDS_2D_IN)
+    dout = dev.alloc_soa(N, clue_hip.FIELDS_2D_OUT)
+    plan = dev.clue2d_plan(din, dout, offs, use_graph=True)
+    assert plan.path == clue_hip.PATH_GLOBAL
+    for rep in range(3):
+        for a in dout.values():
+            a.upload(np.full(N, -7, a.dtype))
+        plan.launch()
+        dev.check()
+        helpers.assert_bit_exact(dev.download_soa(dout), want, helpers.OUT2D, f"plan replay {rep}")
+    plan.destroy()
+    for a in list(din.values()) + list(dout.values()):
+        a.free()
+
+
+def test_set_path_rejects_bad_values(dev):
+    with pytest.raises(clue_hip.ClueError):
+        dev.set_path(7)
+    with pytest.raises(clue_hip.ClueError):
+        dev.set_path(clue_hip.PATH_FUSED)  # disabled on the device
+    with pytest.raises(clue_hip.ClueError):
+        dev.set_path(clue_hip.PATH_FUSED, dim=3)
