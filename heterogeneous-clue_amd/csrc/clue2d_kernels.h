@@ -1,0 +1,395 @@
+// Device code of the CLUE2D global-sort path: key and payload functors of the three radix passes
+// and the density / nearest-higher kernels. Header-only so that the kernels also build for the CPU
+// emulation of the HIP execution model used by the tests; the launch sequence is in clue2d.hip.
+//
+// Reference semantics: src/serial/plugin-CLUEClusterizer/CLUEAlgoKernels.h:7-153.
+// Bit-exactness rests on three facts:
+//  * the sorted array lists each (layer, xBin, yBin) bin in ascending point index, keyed x-major,
+//    so the bins (xBin, yLo..yHi) of one search-box column are one contiguous run, and walking the
+//    columns in ascending xBin visits candidates in exactly the serial order (xBin outer, yBin
+//    inner, index ascending) -> identical fp32 summation order for rho and the same "first
+//    minimum wins" order for delta;
+//  * every expression keeps the reference's operands and order, without FMA contraction;
+//  * sqrtf is correctly rounded (the library is built with correctly rounded fp32 sqrt/div).
+#pragma once
+
+#include <float.h>
+#include <math.h>
+
+#include "clue2d.h"
+
+namespace clue {
+namespace g2d {
+
+struct Keys2D {
+  const float* x;
+  const float* y;
+  const int32_t* layer;
+  uint32_t* keyOut;
+  int* err;
+  __device__ __forceinline__ uint32_t operator()(int i) const {
+    int L = layer[i];
+    if (L < 0 || L >= k2dLayers) {
+      atomicOr(err, kErrLayerRange);
+      L = L < 0 ? 0 : k2dLayers - 1;
+    }
+    const uint32_t k = (uint32_t)(L * k2dBinsPerLayer + xbin2d(x[i]) * k2dRows + ybin2d(y[i]));
+    keyOut[i] = k;
+    return k;
+  }
+};
+
+// Last pass: also gathers x, y, weight into tile order.
+struct EmitLast2D {
+  const int* iIn;
+  uint32_t* kOut;
+  int* iOut;
+  const float* x;
+  const float* y;
+  const float* w;
+  float4* xyw;
+  struct Payload {
+    int idx;
+    float x, y, w;
+  };
+  __device__ __forceinline__ Payload load(int i) const {
+    const int idx = iIn[i];
+    return Payload{idx, x[idx], y[idx], w[idx]};
+  }
+  __device__ __forceinline__ void store(int pos, uint32_t k, const Payload& v) const {
+    kOut[pos] = k;
+    iOut[pos] = v.idx;
+    xyw[pos] = make_float4(v.x, v.y, v.w, 0.f);
+  }
+};
+
+using DigitY = Digit<1, k2dRows>;
+using DigitX = Digit<k2dRows, k2dCols>;
+using DigitL = Digit<k2dBinsPerLayer, kRadix>;
+
+// The density and nearest-higher kernels give each block kCompHits consecutive sorted points of one
+// event. In x-major tile order everything their search boxes can reach is one contiguous range of
+// the sorted array: from the first bin of the lowest box to the end of the highest box. When that
+// range has at most kStage points it is staged into LDS once and every candidate is read from
+// there; otherwise candidates are read from global memory. Either way the candidates are visited
+// in the serial order, so the results do not depend on which path ran.
+constexpr int kCompItems = 2;
+constexpr int kCompHits = kThreads * kCompItems;    // points per compute block
+constexpr int kCompPerSort = kSortBlock / kCompHits;  // compute blocks per sort block
+constexpr int kStage = 1024;                         // staged points (16 KB of float4)
+constexpr int kMaxCols = 5;                          // column runs kept in registers per point
+
+static_assert(kSortBlock % kCompHits == 0, "compute blocks tile each sort block");
+static_assert(kStage % kThreads == 0, "staging splits evenly over the block");
+
+// A search box of the reference (LayerTilesSerial::searchBox) as column runs: run k covers column
+// xlo + k, rows ylo..yhi, which are the sorted positions [s[k], t[k]). Columns beyond kMaxCols
+// (only for large distances) are read from the table when they are scanned.
+struct Runs {
+  int s[kMaxCols], t[kMaxCols];
+  int n, L, xlo, xhi, ylo, yhi;
+};
+
+__device__ __forceinline__ void make_runs(const int* __restrict__ T, uint32_t key, float xi, float yi, float d,
+                                          Runs& r) {
+  r.L = (int)(key / (uint32_t)k2dBinsPerLayer);
+  r.xlo = xbin2d(xi - d);
+  r.xhi = xbin2d(xi + d);
+  r.ylo = ybin2d(yi - d);
+  r.yhi = ybin2d(yi + d);
+  r.n = r.xhi - r.xlo + 1;
+  const int* TL = T + r.L * k2dBinsPerLayer;
+#pragma unroll
+  for (int k = 0; k < kMaxCols; ++k) {
+    const int xb = r.xlo + k < r.xhi ? r.xlo + k : r.xhi;
+    r.s[k] = TL[xb * k2dRows + r.ylo];
+    r.t[k] = TL[xb * k2dRows + r.yhi + 1];
+  }
+}
+
+// First and one-past-last sorted positions the box reaches.
+__device__ __forceinline__ int runs_lo(const Runs& r) { return r.s[0]; }
+__device__ __forceinline__ int runs_hi(const int* __restrict__ T, const Runs& r) {
+  return r.n <= kMaxCols ? r.t[kMaxCols - 1] : T[r.L * k2dBinsPerLayer + r.xhi * k2dRows + r.yhi + 1];
+}
+
+// Block-wide [min lo, max hi).
+__device__ __forceinline__ void block_range(int lo, int hi, int* s_red, int& outLo, int& outHi) {
+  const int lane = lane_id(), wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const int ol = __shfl_xor(lo, off, kWave);
+    const int oh = __shfl_xor(hi, off, kWave);
+    lo = ol < lo ? ol : lo;
+    hi = oh > hi ? oh : hi;
+  }
+  if (lane == 0) {
+    s_red[wave] = lo;
+    s_red[kWaves + wave] = hi;
+  }
+  __syncthreads();
+  outLo = s_red[0];
+  outHi = s_red[kWaves];
+#pragma unroll
+  for (int w = 1; w < kWaves; ++w) {
+    outLo = s_red[w] < outLo ? s_red[w] : outLo;
+    outHi = s_red[kWaves + w] > outHi ? s_red[kWaves + w] : outHi;
+  }
+}
+
+// Copy [lo, hi) (1 <= hi - lo <= kStage points) into LDS, all loads of a thread in flight together
+// (indices past the end re-read the last point and are not stored).
+__device__ __forceinline__ void stage_range(const float4* __restrict__ g, float4* st, int lo, int hi) {
+  constexpr int kPer = kStage / kThreads;
+  const int n = hi - lo;
+  float4 v[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int k = u * kThreads + (int)threadIdx.x;
+    v[u] = g[lo + (k < n ? k : n - 1)];
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int k = u * kThreads + (int)threadIdx.x;
+    if (k < n) st[k] = v[u];
+  }
+}
+
+template <bool STAGED>
+__device__ __forceinline__ float4 cand(const float4* __restrict__ g, const float4* st, int q, int lo) {
+  if constexpr (STAGED)
+    return st[q - lo];
+  else
+    return g[q];
+}
+
+// Run k >= kMaxCols of a box: sorted positions [s, t) of column xlo + k, read from the table.
+__device__ __forceinline__ void run_bounds(const int* __restrict__ T, const Runs& r, int k, int& s, int& t) {
+  const int* TL = T + r.L * k2dBinsPerLayer + (r.xlo + k) * k2dRows;
+  s = TL[r.ylo];
+  t = TL[r.yhi + 1];
+}
+
+// rho of one point (kernel_calculate_density, CLUEAlgoKernels.h:25-55), summed in the serial order.
+// A contribution that fails the distance test adds +0.f, which leaves rho unchanged: rho starts at
+// +0 and a round-to-nearest sum started from +0 is never -0.
+template <bool STAGED>
+__device__ __forceinline__ float density_run(int p, float xi, float yi, int s, int t,
+                                             const float4* __restrict__ xyw, const float4* st, int lo, float dc2,
+                                             float rho) {
+  int q = s;
+  for (; q + 1 < t; q += 2) {
+    const float4 c0 = cand<STAGED>(xyw, st, q, lo);
+    const float4 c1 = cand<STAGED>(xyw, st, q + 1, lo);
+    const float dx0 = xi - c0.x, dy0 = yi - c0.y;
+    const float dx1 = xi - c1.x, dy1 = yi - c1.y;
+    const float d20 = dx0 * dx0 + dy0 * dy0;
+    const float d21 = dx1 * dx1 + dy1 * dy1;
+    const float w0 = (q == p ? 1.f : 0.5f) * c0.z;
+    const float w1 = (q + 1 == p ? 1.f : 0.5f) * c1.z;
+    rho += d20 <= dc2 ? w0 : 0.f;
+    rho += d21 <= dc2 ? w1 : 0.f;
+  }
+  if (q < t) {
+    const float4 c0 = cand<STAGED>(xyw, st, q, lo);
+    const float dx0 = xi - c0.x, dy0 = yi - c0.y;
+    const float d20 = dx0 * dx0 + dy0 * dy0;
+    const float w0 = (q == p ? 1.f : 0.5f) * c0.z;
+    rho += d20 <= dc2 ? w0 : 0.f;
+  }
+  return rho;
+}
+
+__global__ __launch_bounds__(kThreads) void k_density2d(const BlockDesc* __restrict__ blocks,
+                                                        const uint32_t* __restrict__ keyS,
+                                                        const float4* __restrict__ xyw, const int* __restrict__ perm,
+                                                        const int* __restrict__ table, int64_t stride, float dc,
+                                                        float dc2, float4* __restrict__ xyrp,
+                                                        float* __restrict__ rhoOut) {
+  __shared__ float4 s_st[kStage];
+  __shared__ int s_red[2 * kWaves];
+  const BlockDesc bd = blocks[blockIdx.x / kCompPerSort];
+  const int b0 = bd.begin + (int)(blockIdx.x % kCompPerSort) * kCompHits;
+  const int b1 = (b0 + kCompHits) < bd.end ? (b0 + kCompHits) : bd.end;
+  if (b0 >= b1) return;  // uniform over the block
+  const int* T = table + (int64_t)bd.event * stride;
+  Runs runs[kCompItems];
+  float4 me[kCompItems];
+  int lo = 0x7fffffff, hi = -1;
+#pragma unroll
+  for (int r = 0; r < kCompItems; ++r) {
+    const int p = b0 + r * kThreads + (int)threadIdx.x;
+    if (p < b1) {
+      me[r] = xyw[p];
+      make_runs(T, keyS[p], me[r].x, me[r].y, dc, runs[r]);
+      const int l = runs_lo(runs[r]), h = runs_hi(T, runs[r]);
+      lo = l < lo ? l : lo;
+      hi = h > hi ? h : hi;
+    }
+  }
+  int rlo, rhi;
+  block_range(lo, hi, s_red, rlo, rhi);
+  const bool staged = rhi - rlo <= kStage;
+  if (staged) stage_range(xyw, s_st, rlo, rhi);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kCompItems; ++r) {
+    const int p = b0 + r * kThreads + (int)threadIdx.x;
+    if (p < b1) {
+      const float xi = me[r].x, yi = me[r].y;
+      float rho = 0.f;
+#pragma unroll
+      for (int k = 0; k < kMaxCols; ++k)
+        if (k < runs[r].n)
+          rho = staged ? density_run<true>(p, xi, yi, runs[r].s[k], runs[r].t[k], xyw, s_st, rlo, dc2, rho)
+                       : density_run<false>(p, xi, yi, runs[r].s[k], runs[r].t[k], xyw, s_st, rlo, dc2, rho);
+      for (int k = kMaxCols; k < runs[r].n; ++k) {
+        int s, t;
+        run_bounds(T, runs[r], k, s, t);
+        rho = staged ? density_run<true>(p, xi, yi, s, t, xyw, s_st, rlo, dc2, rho)
+                     : density_run<false>(p, xi, yi, s, t, xyw, s_st, rlo, dc2, rho);
+      }
+      const int i = perm[p];
+      xyrp[p] = make_float4(xi, yi, rho, __int_as_float(i));
+      rhoOut[i] = rho;
+    }
+  }
+}
+
+// Nearest higher (kernel_calculate_distanceToHigher, CLUEAlgoKernels.h:57-106): the reference keeps
+// the first candidate, in serial order, with the strictly smallest sqrtf(d2) among the higher ones
+// with sqrtf(d2) <= dm. The scan keeps the running minimum of d2 instead (strict improvements,
+// so the first of equal d2 values) and the running minimum before it. Because sqrtf is monotone
+// (non-decreasing), every candidate before the final minimum has d2 >= that previous minimum; so
+// unless the previous minimum has the same sqrtf as the final one, no earlier candidate can tie
+// with it and the final minimum is the reference's answer. In the tie case the scan is redone
+// exactly as the reference does it.
+template <bool STAGED>
+__device__ __forceinline__ void nh_run(int i, float xi, float yi, float rhoi, int s, int t,
+                                       const float4* __restrict__ xyrp, const float4* st, int lo, float dm2,
+                                       float& best2, int& best, float& prev2, int& prev) {
+  for (int q = s; q < t; ++q) {
+    const float4 c = cand<STAGED>(xyrp, st, q, lo);
+    const float dx = xi - c.x;
+    const float dy = yi - c.y;
+    const float d2 = dx * dx + dy * dy;
+    const int j = __float_as_int(c.w);
+    const bool higher = (c.z > rhoi) || ((c.z == rhoi) && (j > i));
+    const bool upd = higher && d2 <= dm2 && d2 < best2;
+    prev2 = upd ? best2 : prev2;
+    prev = upd ? best : prev;
+    best2 = upd ? d2 : best2;
+    best = upd ? j : best;
+  }
+}
+
+template <bool STAGED>
+__device__ __forceinline__ void nh_run_exact(int i, float xi, float yi, float rhoi, int s, int t,
+                                             const float4* __restrict__ xyrp, const float4* st, int lo, float dm2,
+                                             float& delta, int& nh) {
+  for (int q = s; q < t; ++q) {
+    const float4 c = cand<STAGED>(xyrp, st, q, lo);
+    const float dx = xi - c.x;
+    const float dy = yi - c.y;
+    const float d2 = dx * dx + dy * dy;
+    const int j = __float_as_int(c.w);
+    if (((c.z > rhoi) || ((c.z == rhoi) && (j > i))) && d2 <= dm2) {
+      const float d = sqrtf(d2);
+      if (d < delta) {
+        delta = d;
+        nh = j;
+      }
+    }
+  }
+}
+
+template <bool STAGED>
+__device__ __forceinline__ int nearest_higher_of(int i, float xi, float yi, float rhoi, const int* __restrict__ T,
+                                                 const Runs& runs, const float4* __restrict__ xyrp,
+                                                 const float4* st, int lo, float dm2, float& delta) {
+  float best2 = INFINITY, prev2 = INFINITY;
+  int best = -1, prev = -1;
+#pragma unroll
+  for (int k = 0; k < kMaxCols; ++k)
+    if (k < runs.n) nh_run<STAGED>(i, xi, yi, rhoi, runs.s[k], runs.t[k], xyrp, st, lo, dm2, best2, best, prev2, prev);
+  for (int k = kMaxCols; k < runs.n; ++k) {
+    int s, t;
+    run_bounds(T, runs, k, s, t);
+    nh_run<STAGED>(i, xi, yi, rhoi, s, t, xyrp, st, lo, dm2, best2, best, prev2, prev);
+  }
+  delta = FLT_MAX;
+  if (best < 0) return -1;
+  delta = sqrtf(best2);
+  if (prev < 0 || sqrtf(prev2) != delta) return best;
+  delta = FLT_MAX;
+  int nh = -1;
+  for (int k = 0; k < runs.n; ++k) {
+    int s, t;
+    run_bounds(T, runs, k, s, t);
+    nh_run_exact<STAGED>(i, xi, yi, rhoi, s, t, xyrp, st, lo, dm2, delta, nh);
+  }
+  return nh;
+}
+
+// delta / nearestHigher plus the seed/outlier decision of kernel_findAndAssign_clusters
+// (CLUEAlgoKernels.h:108-137).
+__global__ __launch_bounds__(kThreads) void k_nh2d(const BlockDesc* __restrict__ blocks,
+                                                   const uint32_t* __restrict__ keyS,
+                                                   const float4* __restrict__ xyrp, const int* __restrict__ table,
+                                                   int64_t stride, const int* __restrict__ evOff, float dc, float rhoc,
+                                                   float odf, float dm2, float* __restrict__ deltaOut,
+                                                   int32_t* __restrict__ nhOut, int32_t* __restrict__ isSeedOut,
+                                                   int* __restrict__ parent) {
+  __shared__ float4 s_st[kStage];
+  __shared__ int s_red[2 * kWaves];
+  const BlockDesc bd = blocks[blockIdx.x / kCompPerSort];
+  const int b0 = bd.begin + (int)(blockIdx.x % kCompPerSort) * kCompHits;
+  const int b1 = (b0 + kCompHits) < bd.end ? (b0 + kCompHits) : bd.end;
+  if (b0 >= b1) return;
+  const int* T = table + (int64_t)bd.event * stride;
+  const int evBase = evOff[bd.event];
+  const float dm = odf * dc;
+  Runs runs[kCompItems];
+  float4 me[kCompItems];
+  int lo = 0x7fffffff, hi = -1;
+#pragma unroll
+  for (int r = 0; r < kCompItems; ++r) {
+    const int p = b0 + r * kThreads + (int)threadIdx.x;
+    if (p < b1) {
+      me[r] = xyrp[p];
+      make_runs(T, keyS[p], me[r].x, me[r].y, dm, runs[r]);
+      const int l = runs_lo(runs[r]), h = runs_hi(T, runs[r]);
+      lo = l < lo ? l : lo;
+      hi = h > hi ? h : hi;
+    }
+  }
+  int rlo, rhi;
+  block_range(lo, hi, s_red, rlo, rhi);
+  const bool staged = rhi - rlo <= kStage;
+  if (staged) stage_range(xyrp, s_st, rlo, rhi);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kCompItems; ++r) {
+    const int p = b0 + r * kThreads + (int)threadIdx.x;
+    if (p < b1) {
+      const int i = __float_as_int(me[r].w);
+      const float rhoi = me[r].z;
+      float delta;
+      const int nhg = staged
+                          ? nearest_higher_of<true>(i, me[r].x, me[r].y, rhoi, T, runs[r], xyrp, s_st, rlo, dm2, delta)
+                          : nearest_higher_of<false>(i, me[r].x, me[r].y, rhoi, T, runs[r], xyrp, s_st, rlo, dm2,
+                                                     delta);
+      deltaOut[i] = delta;
+      nhOut[i] = nhg >= 0 ? nhg - evBase : -1;
+      const bool seed = (delta > dc) && (rhoi >= rhoc);
+      const bool outlier = (delta > dm) && (rhoi < rhoc);
+      isSeedOut[i] = seed ? 1 : 0;
+      // Followers point at their nearest higher; seeds, outliers (and a point without a nearest
+      // higher, which the reference cannot register) are roots.
+      parent[i] = (seed || outlier || nhg < 0) ? i : nhg;
+    }
+  }
+}
+
+}  // namespace g2d
+}  // namespace clue

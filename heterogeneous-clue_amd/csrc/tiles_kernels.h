@@ -1,0 +1,176 @@
+// Device code of the tile construction shared by CLUE2D and CLUE3D (device-wide scan, bin table,
+// label propagation, fill). Header-only so that the kernels also build for the CPU emulation of the
+// HIP execution model used by the tests; the launch functions are in tiles.hip.
+#pragma once
+
+#include "tiles.h"
+
+namespace clue {
+
+// ---------------------------------------------------------------- scan
+
+__global__ __launch_bounds__(kThreads) void k_scan_reduce(const int* __restrict__ in, int64_t n,
+                                                          int* __restrict__ part) {
+  __shared__ int s_tmp[kWaves];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int v = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k)
+    if (base + k < n) v += in[base + k];
+  int total;
+  block_exclusive_sum(v, s_tmp, total);
+  if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+// Single block: exclusive scan of the partials, chunk by chunk with a carry.
+__global__ __launch_bounds__(kThreads) void k_scan_partials(int* __restrict__ part, int np) {
+  __shared__ int s_tmp[kWaves];
+  int carry = 0;
+  for (int c = 0; c < np; c += kScanTile) {
+    const int base = c + (int)threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      v[k] = (base + k < np) ? part[base + k] : 0;
+      sum += v[k];
+    }
+    int total;
+    int run = carry + block_exclusive_sum(sum, s_tmp, total);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      if (base + k < np) part[base + k] = run;
+      run += v[k];
+    }
+    carry += total;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_down(const int* in, int* out, int64_t n,
+                                                        const int* __restrict__ part) {
+  __shared__ int s_tmp[kWaves];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int v[kScanItems];
+  int sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = (base + k < n) ? in[base + k] : 0;
+    sum += v[k];
+  }
+  int total;
+  int run = part[blockIdx.x] + block_exclusive_sum(sum, s_tmp, total);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+}
+
+
+// ---------------------------------------------------------------- dense bin table
+
+constexpr int kTablePer = kTableTile / kThreads;  // bins per thread
+
+// tileStart[e][k] = first sorted position of event e whose key is >= k * kTableTile, k in [0, tiles].
+__global__ __launch_bounds__(kThreads) void k_tile_starts(const BlockDesc* __restrict__ blocks,
+                                                          const uint32_t* __restrict__ key,
+                                                          const int* __restrict__ evOff, int tiles,
+                                                          int* __restrict__ tileStart) {
+  const BlockDesc bd = blocks[blockIdx.x];
+  const int s = evOff[bd.event], t = evOff[bd.event + 1];
+  int* TS = tileStart + (int64_t)bd.event * (tiles + 1);
+#pragma unroll 1
+  for (int r = 0; r < kSortItems; ++r) {
+    const int p = bd.begin + r * kThreads + (int)threadIdx.x;
+    if (p >= bd.end) break;
+    const int tl = (int)(key[p] / (uint32_t)kTableTile);
+    const int prev = (p == s) ? -1 : (int)(key[p - 1] / (uint32_t)kTableTile);
+    for (int k = prev + 1; k <= tl; ++k) TS[k] = p;
+    if (p == t - 1)
+      for (int k = tl + 1; k <= tiles; ++k) TS[k] = t;
+  }
+}
+
+// LDS index of bin j of a tile, padded by one word per kTablePer so that the per-thread chunks
+// [16t, 16t+16) fall into distinct banks.
+__device__ __forceinline__ int tpad(int j) { return j + j / kTablePer; }
+
+__global__ __launch_bounds__(kThreads) void k_table(const uint32_t* __restrict__ key, const int* __restrict__ evOff,
+                                                    const int* __restrict__ tileStart, int tiles, int nKeys,
+                                                    int64_t stride, int* __restrict__ table) {
+  __shared__ int s_first[kTableTile + kTableTile / kTablePer];
+  __shared__ int s_tmp[kWaves];
+  const int e = blockIdx.x / tiles;
+  const int tile = blockIdx.x % tiles;
+  const int b0 = tile * kTableTile;
+  const int b1 = (b0 + kTableTile) < (nKeys + 1) ? (b0 + kTableTile) : (nKeys + 1);
+  const int s = evOff[e], t = evOff[e + 1];
+  int lo = s, hi = s;
+  if (t > s) {  // tile starts are only recorded for non-empty events
+    const int* TS = tileStart + (int64_t)e * (tiles + 1);
+    lo = TS[tile];
+    hi = TS[tile + 1];
+  }
+  for (int j = threadIdx.x; j < kTableTile; j += kThreads) s_first[tpad(j)] = 0x7fffffff;
+  __syncthreads();
+  // first position of every non-empty bin of the tile (one writer per bin: the run start)
+  for (int p = lo + (int)threadIdx.x; p < hi; p += kThreads) {
+    const uint32_t k = key[p];
+    if (p == lo || key[p - 1] != k) s_first[tpad((int)(k - (uint32_t)b0))] = p;
+  }
+  __syncthreads();
+  // table[b] = min(first of bins >= b, hi): a suffix minimum. Thread t owns bins
+  // [(kThreads-1-t)*kTablePer, +kTablePer) of the tile, so thread order runs from high bins to low.
+  const int j0 = (kThreads - 1 - (int)threadIdx.x) * kTablePer;
+  int m = 0x7fffffff;
+#pragma unroll
+  for (int k = 0; k < kTablePer; ++k) {
+    const int v = s_first[tpad(j0 + k)];
+    m = v < m ? v : m;
+  }
+  int run = block_exclusive_min(m, s_tmp);
+  run = run < hi ? run : hi;
+#pragma unroll
+  for (int k = kTablePer - 1; k >= 0; --k) {
+    const int v = s_first[tpad(j0 + k)];
+    run = v < run ? v : run;
+    s_first[tpad(j0 + k)] = run;
+  }
+  __syncthreads();
+  int* out = table + (int64_t)e * stride + b0;
+  for (int j = threadIdx.x; j < b1 - b0; j += kThreads) out[j] = s_first[tpad(j)];
+}
+
+
+// ---------------------------------------------------------------- labels
+
+__global__ __launch_bounds__(kThreads) void k_labels(const BlockDesc* __restrict__ blocks, int* parent,
+                                                     const int* __restrict__ isSeed, const int* __restrict__ rankAll,
+                                                     const int* __restrict__ evOff, int32_t* __restrict__ label) {
+  const BlockDesc bd = blocks[blockIdx.x];
+  const int evBase = evOff[bd.event];
+  const int rank0 = rankAll[evBase];
+#pragma unroll 1
+  for (int r = 0; r < kSortItems; ++r) {
+    const int i = bd.begin + r * kThreads + (int)threadIdx.x;
+    if (i >= bd.end) break;
+    // Walk to the root. Every value read from parent[] (old or already shortened by another
+    // thread) is an ancestor of the point read, and roots never change, so the walk ends.
+    int cur = parent[i];
+    for (;;) {
+      const int nxt = parent[cur];
+      if (nxt == cur) break;
+      cur = nxt;
+    }
+    parent[i] = cur;
+    label[i] = isSeed[cur] ? rankAll[cur] - rank0 : -1;
+  }
+}
+
+
+__global__ void k_fill(int* p, int v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace clue

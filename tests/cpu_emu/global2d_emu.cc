@@ -1,0 +1,73 @@
+// Runs the CLUE2D global-sort path (three radix passes, bin table, density, nearest higher, seed
+// ranks, labels) on the CPU through the HIP-model emulation, with clue2d_enqueue's launch sequence
+// and grids. Test infrastructure only. usage: global2d_emu <in> <out> (formats: launch_emu.h)
+#include "clue2d_kernels.h"
+#include "launch_emu.h"
+
+using namespace clue;
+using namespace clue::g2d;
+using namespace emu_host;
+
+int main(int argc, char** argv) {
+  if (argc < 3) return 2;
+  Input2D in = read_input(argv[1]);
+  const int E = in.E;
+  const int N = (int)in.offs[E];
+  std::vector<BlockDesc> blocks = blocks_of(in.offs);
+  const int nb = (int)blocks.size();
+  std::vector<int> evOff(E + 1);
+  for (int e = 0; e <= E; ++e) evOff[e] = (int)in.offs[e];
+  const int64_t stride = clue2d_table_stride();
+  const int tiles = table_tiles(k2dKeys);
+  // device scratch starts with arbitrary contents
+  const int P = 0x5A5A5A5A;
+  std::vector<uint32_t> k1(N, P), k2(N, P);
+  std::vector<int> i1(N, P), i2(N, P), hist((size_t)kRadix * std::max(nb, 1), P);
+  const int64_t maxScan = std::max<int64_t>((int64_t)kRadix * nb, N);
+  std::vector<int> part(scan_partials(std::max<int64_t>(maxScan, 1)), P);
+  std::vector<int> table((size_t)stride * std::max(E, 1), P), tileStart((size_t)(tiles + 1) * std::max(E, 1), P);
+  std::vector<float4> xyw(N, float4{NAN, NAN, NAN, NAN}), xyrp(N, float4{NAN, NAN, NAN, NAN});
+  std::vector<float> rho(N, -7.f), delta(N, -7.f);
+  std::vector<int> nh(N, -7), isSeed(N, -7), ci(N, -7);
+  int err = 0;
+  const float dc = in.dc, rhoc = in.rhoc, odf = in.odf;
+  const float dc2 = sqrt_le(dc), dm2 = sqrt_le(odf * dc);
+  if (nb > 0) {
+    const int nHist = kRadix * nb;
+    Keys2D keys{in.x.data(), in.y.data(), in.layer.data(), k2.data(), &err};
+    emu::launch(nb, kThreads, [&] { k_radix_hist<Keys2D, DigitY>(blocks.data(), keys, hist.data()); });
+    scan_exclusive(hist.data(), hist.data(), nHist, part.data());
+    emu::launch(nb, kThreads, [&] {
+      k_radix_scatter<KeyArray, DigitY, EmitFirst>(blocks.data(), KeyArray{k2.data()}, hist.data(),
+                                                   EmitFirst{k1.data(), i1.data()});
+    });
+    emu::launch(nb, kThreads, [&] { k_radix_hist<KeyArray, DigitX>(blocks.data(), KeyArray{k1.data()}, hist.data()); });
+    scan_exclusive(hist.data(), hist.data(), nHist, part.data());
+    emu::launch(nb, kThreads, [&] {
+      k_radix_scatter<KeyArray, DigitX, EmitMid>(blocks.data(), KeyArray{k1.data()}, hist.data(),
+                                                 EmitMid{i1.data(), k2.data(), i2.data()});
+    });
+    emu::launch(nb, kThreads, [&] { k_radix_hist<KeyArray, DigitL>(blocks.data(), KeyArray{k2.data()}, hist.data()); });
+    scan_exclusive(hist.data(), hist.data(), nHist, part.data());
+    emu::launch(nb, kThreads, [&] {
+      k_radix_scatter<KeyArray, DigitL, EmitLast2D>(
+          blocks.data(), KeyArray{k2.data()}, hist.data(),
+          EmitLast2D{i2.data(), k1.data(), i1.data(), in.x.data(), in.y.data(), in.w.data(), xyw.data()});
+    });
+    build_table(k1.data(), blocks.data(), nb, evOff.data(), E, k2dKeys, stride, table.data(), tileStart.data());
+    emu::launch(kCompPerSort * nb, kThreads, [&] {
+      k_density2d(blocks.data(), k1.data(), xyw.data(), i1.data(), table.data(), stride, dc, dc2, xyrp.data(),
+                  rho.data());
+    });
+    int* parent = reinterpret_cast<int*>(k2.data());
+    int* rankAll = i2.data();
+    emu::launch(kCompPerSort * nb, kThreads, [&] {
+      k_nh2d(blocks.data(), k1.data(), xyrp.data(), table.data(), stride, evOff.data(), dc, rhoc, odf, dm2,
+             delta.data(), nh.data(), isSeed.data(), parent);
+    });
+    scan_exclusive(isSeed.data(), rankAll, N, part.data());
+    launch_labels(blocks.data(), nb, parent, isSeed.data(), rankAll, evOff.data(), ci.data());
+  }
+  write_output(argv[2], err, 0, rho, delta, nh, isSeed, ci);
+  return 0;
+}

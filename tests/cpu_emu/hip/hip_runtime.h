@@ -81,11 +81,29 @@ inline Group* group = nullptr;
 
 inline Wave& my_wave() { return group->waves[tIdx.x / kWave]; }
 
-// Run kernel(args...) over grid x block threads, one workgroup at a time.
+// Run body() over grid x block threads, one workgroup at a time; the block's threads are created
+// once per launch and step through the workgroups together.
 template <class F>
 void launch(unsigned grid, unsigned block, F&& body) {
   gDim = dim3{grid, 1, 1};
   bDim = dim3{block, 1, 1};
+  if (grid == 0) return;
+  std::barrier<> start(block + 1), done(block + 1);
+  std::vector<std::thread> ts;
+  ts.reserve(block);
+  for (unsigned t = 0; t < block; ++t)
+    ts.emplace_back([&, t] {
+      tIdx = dim3{t, 0, 0};
+      for (unsigned b = 0; b < grid; ++b) {
+        start.arrive_and_wait();
+        bIdx = dim3{b, 0, 0};
+        body();
+        // a thread that has left the kernel no longer takes part in barriers
+        my_wave().bar->arrive_and_drop();
+        group->bar->arrive_and_drop();
+        done.arrive_and_wait();
+      }
+    });
   for (unsigned b = 0; b < grid; ++b) {
     Group g;
     g.bar = std::make_unique<std::barrier<>>(block);
@@ -96,20 +114,11 @@ void launch(unsigned grid, unsigned block, F&& body) {
       g.waves[w].bar = std::make_unique<std::barrier<>>(lanes);
     }
     group = &g;
-    std::vector<std::thread> ts;
-    ts.reserve(block);
-    for (unsigned t = 0; t < block; ++t)
-      ts.emplace_back([&, t] {
-        tIdx = dim3{t, 0, 0};
-        bIdx = dim3{b, 0, 0};
-        body();
-        // a thread that has left the kernel no longer takes part in barriers
-        my_wave().bar->arrive_and_drop();
-        group->bar->arrive_and_drop();
-      });
-    for (auto& t : ts) t.join();
+    start.arrive_and_wait();
+    done.arrive_and_wait();
     group = nullptr;
   }
+  for (auto& t : ts) t.join();
 }
 
 template <class T>

@@ -1,9 +1,13 @@
-"""The fused CLUE2D kernels (layer bucket pass, segment lists, fused segment kernels, seed numbering)
-run on the CPU through an emulation of the HIP execution model (one thread per lane, real barriers,
-wave ballots and shuffles, AddressSanitizer), bit-exact against the golden fixtures and the oracle.
+"""The library's CLUE2D kernels run on the CPU through an emulation of the HIP execution model (one
+thread per lane, real barriers, wave ballots and shuffles through a per-wave mailbox,
+AddressSanitizer + UndefinedBehaviorSanitizer), with the library's own launch sequences and grids,
+bit-exact against the golden fixtures and the oracle:
 
-This pins the fused path's device code on the CPU; the library keeps that path disabled on the device
-(clue_set_path refuses it) until it has also passed on the GPU."""
+* the global-sort path (radix passes, bin table, density, nearest higher, seed ranks, labels), the
+  path the device runs; device scratch starts poisoned, so reads of unwritten memory show up;
+* the fused segment path (layer bucket pass, segment lists, fused segment kernels, seed numbering),
+  which the library keeps disabled on the device (clue_set_path refuses it) until it has also
+  passed on the GPU."""
 import subprocess
 from pathlib import Path
 
@@ -15,6 +19,7 @@ import synth
 
 EMU = Path(__file__).resolve().parent / "cpu_emu"
 EXE = EMU / "_build" / "fused2d_emu"
+EXE_GLOBAL = EMU / "_build" / "global2d_emu"
 
 
 @pytest.fixture(scope="module")
@@ -86,3 +91,21 @@ def test_emulated_capacity_is_flagged(emulator, tmp_path):
           "layer": np.zeros(n, np.int32), "weight": np.ones(n, np.float32)}
     err, big, _ = run_emulated(emulator, tmp_path, ev, np.array([0, n]))
     assert big == n and err & 4
+
+
+@pytest.mark.parametrize("name", ["grid_ties", "bin_edges"])
+def test_emulated_global_golden(emulator, tmp_path, name):
+    ins, want, extra = helpers.load_case("clue2d", name)
+    dc, rhoc, odf = (float(v) for v in extra["params"])
+    err, _, got = run_emulated(EXE_GLOBAL, tmp_path, ins, np.array([0, ins["x"].size]), dc, rhoc, odf)
+    assert err == 0
+    helpers.assert_bit_exact(got, want, helpers.OUT2D, f"emulated global {name}")
+
+
+def test_emulated_global_batch(emulator, tmp_path, oracle_lib):
+    """Three events (one empty, one spanning two sort blocks) in one batch."""
+    evs = [synth.clue2d_event(n, 70 + i) for i, n in enumerate([2500, 0, 700])]
+    packed, offs = synth.pack(evs)
+    err, _, got = run_emulated(EXE_GLOBAL, tmp_path, packed, offs)
+    assert err == 0
+    helpers.assert_bit_exact(got, oracle_lib.clue2d_batch(packed, offs), helpers.OUT2D, "emulated global batch")
