@@ -170,6 +170,48 @@ __device__ __forceinline__ void run_bounds(const int* __restrict__ T, const Runs
   t = TL[r.yhi + 1];
 }
 
+// Visits the candidates of the register runs [s[k], t[k]), k < min(n, kMaxCols), in column order,
+// calling f(q) once per candidate. One loop iteration per candidate of this lane, moving to the next
+// non-empty run only when the current one ends: a wave then iterates the largest candidate count of
+// its lanes, instead of the sum over columns of each column's largest run.
+static_assert(kMaxCols == 5, "walk_runs keeps the pending runs in five register pairs");
+template <class F>
+__device__ __forceinline__ void walk_runs(const Runs& r, F& f) {
+  int pend = (r.n < kMaxCols ? r.n : kMaxCols) - 1;
+  int q = r.s[0], t = r.t[0];
+  int a1 = r.s[1], b1 = r.t[1], a2 = r.s[2], b2 = r.t[2], a3 = r.s[3], b3 = r.t[3], a4 = r.s[4], b4 = r.t[4];
+  while (q >= t && pend > 0) {
+    q = a1; t = b1; a1 = a2; b1 = b2; a2 = a3; b2 = b3; a3 = a4; b3 = b4; --pend;
+  }
+  while (q < t) {
+    f(q);
+    ++q;
+    if (q >= t) {
+      while (q >= t && pend > 0) {
+        q = a1; t = b1; a1 = a2; b1 = b2; a2 = a3; b2 = b3; a3 = a4; b3 = b4; --pend;
+      }
+    }
+  }
+}
+
+// One candidate's rho contribution (CLUEAlgoKernels.h:36-52); see density_run.
+template <bool STAGED>
+struct DensityStep {
+  int p;
+  float xi, yi, dc2;
+  const float4* g;
+  const float4* st;
+  int lo;
+  float rho;
+  __device__ __forceinline__ void operator()(int q) {
+    const float4 c = cand<STAGED>(g, st, q, lo);
+    const float dx = xi - c.x, dy = yi - c.y;
+    const float d2 = dx * dx + dy * dy;
+    const float w = (q == p ? 1.f : 0.5f) * c.z;
+    rho += d2 <= dc2 ? w : 0.f;
+  }
+};
+
 // rho of one point (kernel_calculate_density, CLUEAlgoKernels.h:25-55), summed in the serial order.
 // A contribution that fails the distance test adds +0.f, which leaves rho unchanged: rho starts at
 // +0 and a round-to-nearest sum started from +0 is never -0.
@@ -238,11 +280,15 @@ __global__ __launch_bounds__(kThreads) void k_density2d(const BlockDesc* __restr
     if (p < b1) {
       const float xi = me[r].x, yi = me[r].y;
       float rho = 0.f;
-#pragma unroll
-      for (int k = 0; k < kMaxCols; ++k)
-        if (k < runs[r].n)
-          rho = staged ? density_run<true>(p, xi, yi, runs[r].s[k], runs[r].t[k], xyw, s_st, rlo, dc2, rho)
-                       : density_run<false>(p, xi, yi, runs[r].s[k], runs[r].t[k], xyw, s_st, rlo, dc2, rho);
+      if (staged) {
+        DensityStep<true> f{p, xi, yi, dc2, xyw, s_st, rlo, 0.f};
+        walk_runs(runs[r], f);
+        rho = f.rho;
+      } else {
+        DensityStep<false> f{p, xi, yi, dc2, xyw, s_st, rlo, 0.f};
+        walk_runs(runs[r], f);
+        rho = f.rho;
+      }
       for (int k = kMaxCols; k < runs[r].n; ++k) {
         int s, t;
         run_bounds(T, runs[r], k, s, t);
@@ -283,6 +329,31 @@ __device__ __forceinline__ void nh_run(int i, float xi, float yi, float rhoi, in
   }
 }
 
+// One candidate of the deferred nearest-higher scan; see nh_run.
+template <bool STAGED>
+struct NhStep {
+  int i;
+  float xi, yi, rhoi, dm2;
+  const float4* g;
+  const float4* st;
+  int lo;
+  float best2, prev2;
+  int best, prev;
+  __device__ __forceinline__ void operator()(int q) {
+    const float4 c = cand<STAGED>(g, st, q, lo);
+    const float dx = xi - c.x;
+    const float dy = yi - c.y;
+    const float d2 = dx * dx + dy * dy;
+    const int j = __float_as_int(c.w);
+    const bool higher = (c.z > rhoi) || ((c.z == rhoi) && (j > i));
+    const bool upd = higher && d2 <= dm2 && d2 < best2;
+    prev2 = upd ? best2 : prev2;
+    prev = upd ? best : prev;
+    best2 = upd ? d2 : best2;
+    best = upd ? j : best;
+  }
+};
+
 template <bool STAGED>
 __device__ __forceinline__ void nh_run_exact(int i, float xi, float yi, float rhoi, int s, int t,
                                              const float4* __restrict__ xyrp, const float4* st, int lo, float dm2,
@@ -307,11 +378,10 @@ template <bool STAGED>
 __device__ __forceinline__ int nearest_higher_of(int i, float xi, float yi, float rhoi, const int* __restrict__ T,
                                                  const Runs& runs, const float4* __restrict__ xyrp,
                                                  const float4* st, int lo, float dm2, float& delta) {
-  float best2 = INFINITY, prev2 = INFINITY;
-  int best = -1, prev = -1;
-#pragma unroll
-  for (int k = 0; k < kMaxCols; ++k)
-    if (k < runs.n) nh_run<STAGED>(i, xi, yi, rhoi, runs.s[k], runs.t[k], xyrp, st, lo, dm2, best2, best, prev2, prev);
+  NhStep<STAGED> f{i, xi, yi, rhoi, dm2, xyrp, st, lo, INFINITY, INFINITY, -1, -1};
+  walk_runs(runs, f);
+  float best2 = f.best2, prev2 = f.prev2;
+  int best = f.best, prev = f.prev;
   for (int k = kMaxCols; k < runs.n; ++k) {
     int s, t;
     run_bounds(T, runs, k, s, t);
