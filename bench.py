@@ -137,6 +137,20 @@ def serving_leg(dev, rate, seconds, seed=11):  # noqa: C901
     return out
 
 
+def copy_probe(nbytes):
+    """Three device-to-device copies of nbytes each: the reference kernel whose FETCH_SIZE / WRITE_SIZE
+    counters scale the counters of the clustering kernels to bytes."""
+    dev = clue_hip.Device(0)
+    a, b = dev.array(nbytes // 4, np.int32), dev.array(nbytes // 4, np.int32)
+    for _ in range(3):
+        dev._chk(dev.lib.clue_memcpy_dtod(dev.ctx, b.ptr, a.ptr, nbytes, None))
+    dev.sync()
+    print(json.dumps({"copy_probe_bytes": nbytes, "copies": 3}), flush=True)
+    a.free()
+    b.free()
+    dev.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,11 +163,17 @@ def main():
     ap.add_argument("--serve-rates", default="500,1500", help="serving leg: event rates (events/s), comma-separated")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="serving leg: length of the stream")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run 3 device-to-device copies of BYTES (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); events are independent, so N GPUs "
               f"would each run their own batch. Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return
 
     counts = synth.c3_hit_counts(args.events)
     packed, offs = synth.clue2d_batch(counts)

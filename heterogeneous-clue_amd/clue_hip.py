@@ -24,6 +24,7 @@ EXPORTS = [
     "clue3d_default_params", "clue_create", "clue_destroy", "clue_last_error", "clue_device_count",
     "clue_bytes_in_use", "clue_stream_create", "clue_stream_destroy", "clue_stream_sync", "clue_malloc_device",
     "clue_free_device", "clue_malloc_pinned", "clue_free_pinned", "clue_memcpy_htod", "clue_memcpy_dtoh",
+    "clue_memcpy_dtod",
     "clue_check", "clue_mark", "clue_elapsed_ms", "clue2d_run_device", "clue2d_run_host",
     "clue2d_run_batch_device", "clue2d_plan_create", "clue3d_run_device", "clue3d_run_host",
     "clue3d_run_batch_device", "clue3d_plan_create", "clue_plan_launch", "clue_plan_destroy",
@@ -82,6 +83,7 @@ def load_library(path: Path = LIB_PATH) -> C.CDLL:
         "clue_free_pinned": (I32, [VP, VP]),
         "clue_memcpy_htod": (I32, [VP, VP, VP, C.c_uint64, VP]),
         "clue_memcpy_dtoh": (I32, [VP, VP, VP, C.c_uint64, VP]),
+        "clue_memcpy_dtod": (I32, [VP, VP, VP, C.c_uint64, VP]),
         "clue_check": (I32, [VP, VP]),
         "clue_mark": (I32, [VP, I32, VP]),
         "clue_elapsed_ms": (C.c_float, [VP, I32, I32]),

@@ -452,6 +452,13 @@ int32_t clue_memcpy_dtoh(clue_ctx* c, void* dst, const void* src, uint64_t bytes
   return CLUE_OK;
 }
 
+int32_t clue_memcpy_dtod(clue_ctx* c, void* dst, const void* src, uint64_t bytes, void* s) {
+  if (!c || (!dst && bytes) || (!src && bytes)) return CLUE_ERR_ARG;
+  if (!bytes) return CLUE_OK;
+  HIP_TRY(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, pick(c, s)));
+  return CLUE_OK;
+}
+
 int32_t clue_check(clue_ctx* c, void* s) {
   if (!c) return CLUE_ERR_ARG;
   hipStream_t hs = pick(c, s);

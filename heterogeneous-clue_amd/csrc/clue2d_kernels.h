@@ -320,8 +320,9 @@ __device__ __forceinline__ void nh_run(int i, float xi, float yi, float rhoi, in
     const float dy = yi - c.y;
     const float d2 = dx * dx + dy * dy;
     const int j = __float_as_int(c.w);
-    const bool higher = (c.z > rhoi) || ((c.z == rhoi) && (j > i));
-    const bool upd = higher && d2 <= dm2 && d2 < best2;
+    // bitwise forms: lane masks combined without branches
+    const bool higher = (c.z > rhoi) | ((c.z == rhoi) & (j > i));
+    const bool upd = higher & (d2 <= dm2) & (d2 < best2);
     prev2 = upd ? best2 : prev2;
     prev = upd ? best : prev;
     best2 = upd ? d2 : best2;
@@ -345,8 +346,9 @@ struct NhStep {
     const float dy = yi - c.y;
     const float d2 = dx * dx + dy * dy;
     const int j = __float_as_int(c.w);
-    const bool higher = (c.z > rhoi) || ((c.z == rhoi) && (j > i));
-    const bool upd = higher && d2 <= dm2 && d2 < best2;
+    // bitwise forms: lane masks combined without branches
+    const bool higher = (c.z > rhoi) | ((c.z == rhoi) & (j > i));
+    const bool upd = higher & (d2 <= dm2) & (d2 < best2);
     prev2 = upd ? best2 : prev2;
     prev = upd ? best : prev;
     best2 = upd ? d2 : best2;

@@ -100,6 +100,7 @@ int32_t clue_free_pinned(clue_ctx* ctx, void* ptr);
 /* Asynchronous on `stream` (stream-ordered); pinned host memory makes them truly asynchronous. */
 int32_t clue_memcpy_htod(clue_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream);
 int32_t clue_memcpy_dtoh(clue_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream);
+int32_t clue_memcpy_dtod(clue_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream);
 
 /* Synchronise `stream` and report errors the kernels flagged since the last check
  * (CLUE_ERR_LAYER, CLUE_ERR_BIN, CLUE_ERR_CAPACITY). */
