@@ -174,6 +174,8 @@ __device__ __forceinline__ void run_bounds(const int* __restrict__ T, const Runs
 // calling f(q) once per candidate. One loop iteration per candidate of this lane, moving to the next
 // non-empty run only when the current one ends: a wave then iterates the largest candidate count of
 // its lanes, instead of the sum over columns of each column's largest run.
+// Candidates are taken two at a time inside a run (the second masked off at an odd run end), so
+// the end-of-run test and the loop branch are paid once per pair and two LDS reads are in flight.
 static_assert(kMaxCols == 5, "walk_runs keeps the pending runs in five register pairs");
 template <class F>
 __device__ __forceinline__ void walk_runs(const Runs& r, F& f) {
@@ -184,8 +186,9 @@ __device__ __forceinline__ void walk_runs(const Runs& r, F& f) {
     q = a1; t = b1; a1 = a2; b1 = b2; a2 = a3; b2 = b3; a3 = a4; b3 = b4; --pend;
   }
   while (q < t) {
-    f(q);
-    ++q;
+    const bool two = q + 1 < t;
+    f(q, two ? q + 1 : q, two);
+    q += two ? 2 : 1;
     if (q >= t) {
       while (q >= t && pend > 0) {
         q = a1; t = b1; a1 = a2; b1 = b2; a2 = a3; b2 = b3; a3 = a4; b3 = b4; --pend;
@@ -194,7 +197,8 @@ __device__ __forceinline__ void walk_runs(const Runs& r, F& f) {
   }
 }
 
-// One candidate's rho contribution (CLUEAlgoKernels.h:36-52); see density_run.
+// Candidates q0 and (if two) q1 = q0 + 1, in that order: their rho contributions
+// (CLUEAlgoKernels.h:36-52); see density_run. A masked or failing term adds +0.f.
 template <bool STAGED>
 struct DensityStep {
   int p;
@@ -203,12 +207,17 @@ struct DensityStep {
   const float4* st;
   int lo;
   float rho;
-  __device__ __forceinline__ void operator()(int q) {
-    const float4 c = cand<STAGED>(g, st, q, lo);
-    const float dx = xi - c.x, dy = yi - c.y;
-    const float d2 = dx * dx + dy * dy;
-    const float w = (q == p ? 1.f : 0.5f) * c.z;
-    rho += d2 <= dc2 ? w : 0.f;
+  __device__ __forceinline__ void operator()(int q0, int q1, bool two) {
+    const float4 c0 = cand<STAGED>(g, st, q0, lo);
+    const float4 c1 = cand<STAGED>(g, st, q1, lo);
+    const float dx0 = xi - c0.x, dy0 = yi - c0.y;
+    const float dx1 = xi - c1.x, dy1 = yi - c1.y;
+    const float d20 = dx0 * dx0 + dy0 * dy0;
+    const float d21 = dx1 * dx1 + dy1 * dy1;
+    const float w0 = (q0 == p ? 1.f : 0.5f) * c0.z;
+    const float w1 = (q1 == p ? 1.f : 0.5f) * c1.z;
+    rho += d20 <= dc2 ? w0 : 0.f;
+    rho += (two & (d21 <= dc2)) ? w1 : 0.f;
   }
 };
 
@@ -330,7 +339,8 @@ __device__ __forceinline__ void nh_run(int i, float xi, float yi, float rhoi, in
   }
 }
 
-// One candidate of the deferred nearest-higher scan; see nh_run.
+// Candidates q0 and (if two) q1 = q0 + 1, in that order, of the deferred nearest-higher scan; see
+// nh_run.
 template <bool STAGED>
 struct NhStep {
   int i;
@@ -340,19 +350,24 @@ struct NhStep {
   int lo;
   float best2, prev2;
   int best, prev;
-  __device__ __forceinline__ void operator()(int q) {
-    const float4 c = cand<STAGED>(g, st, q, lo);
+  __device__ __forceinline__ void take(const float4& c, bool valid) {
     const float dx = xi - c.x;
     const float dy = yi - c.y;
     const float d2 = dx * dx + dy * dy;
     const int j = __float_as_int(c.w);
     // bitwise forms: lane masks combined without branches
     const bool higher = (c.z > rhoi) | ((c.z == rhoi) & (j > i));
-    const bool upd = higher & (d2 <= dm2) & (d2 < best2);
+    const bool upd = valid & higher & (d2 <= dm2) & (d2 < best2);
     prev2 = upd ? best2 : prev2;
     prev = upd ? best : prev;
     best2 = upd ? d2 : best2;
     best = upd ? j : best;
+  }
+  __device__ __forceinline__ void operator()(int q0, int q1, bool two) {
+    const float4 c0 = cand<STAGED>(g, st, q0, lo);
+    const float4 c1 = cand<STAGED>(g, st, q1, lo);
+    take(c0, true);
+    take(c1, two);
   }
 };
 

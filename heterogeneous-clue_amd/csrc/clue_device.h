@@ -83,6 +83,18 @@ __device__ __forceinline__ int phi_bin3d(float phi) {
   return d2i_x86(((double)normPhi + 3.14159265358979323846) * (double)r);
 }
 
+// ---------------------------------------------------------------- workgroup placement
+// Workgroups are handed to the 8 XCDs of the chip round-robin (workgroup b runs on XCD b % 8), and
+// each XCD has its own L2. xcd_block() renumbers the grid so that XCD x works through one contiguous
+// range of logical blocks - neighbouring sort blocks, hence the same events' data, share an L2.
+// A bijection of [0, G).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int G) {
+  const int x = b % kXcds, k = b / kXcds;
+  const int q = G / kXcds, r = G % kXcds;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 // ---------------------------------------------------------------- wave / block primitives
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 

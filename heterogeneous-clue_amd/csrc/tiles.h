@@ -65,7 +65,7 @@ template <class Src, class D>
 __global__ __launch_bounds__(kThreads) void k_radix_hist(const BlockDesc* __restrict__ blocks, Src src,
                                                          int* __restrict__ hist) {
   __shared__ int s_h[kRadix];
-  const BlockDesc bd = blocks[blockIdx.x];
+  const BlockDesc bd = blocks[xcd_block((int)blockIdx.x, (int)gridDim.x)];
   if (threadIdx.x < kRadix) s_h[threadIdx.x] = 0;
   // all of the thread's keys in flight at once
   int d[kSortItems];
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const BlockDesc* __r
   __shared__ int s_off[kWaves][kRadix];
   __shared__ uint32_t s_key[kSortBlock];
   __shared__ Payload s_pay[kSortBlock];
-  const BlockDesc bd = blocks[blockIdx.x];
+  const BlockDesc bd = blocks[xcd_block((int)blockIdx.x, (int)gridDim.x)];
   const int tid = (int)threadIdx.x, lane = lane_id(), wave = tid / kWave;
   const int nItems = bd.end - bd.begin;
   uint32_t key[kSortItems];

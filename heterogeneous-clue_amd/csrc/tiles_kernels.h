@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kThreads) void k_tile_starts(const BlockDesc* __res
                                                           const uint32_t* __restrict__ key,
                                                           const int* __restrict__ evOff, int tiles,
                                                           int* __restrict__ tileStart) {
-  const BlockDesc bd = blocks[blockIdx.x];
+  const BlockDesc bd = blocks[xcd_block((int)blockIdx.x, (int)gridDim.x)];
   const int s = evOff[bd.event], t = evOff[bd.event + 1];
   int* TS = tileStart + (int64_t)bd.event * (tiles + 1);
 #pragma unroll 1
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(kThreads) void k_table(const uint32_t* __restrict__
 __global__ __launch_bounds__(kThreads) void k_labels(const BlockDesc* __restrict__ blocks, int* parent,
                                                      const int* __restrict__ isSeed, const int* __restrict__ rankAll,
                                                      const int* __restrict__ evOff, int32_t* __restrict__ label) {
-  const BlockDesc bd = blocks[blockIdx.x];
+  const BlockDesc bd = blocks[xcd_block((int)blockIdx.x, (int)gridDim.x)];
   const int evBase = evOff[bd.event];
   const int rank0 = rankAll[evBase];
 #pragma unroll 1
