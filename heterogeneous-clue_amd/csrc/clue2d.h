@@ -41,6 +41,7 @@ struct Clue2DBuffers {
   int* table = nullptr;
   int* tileStart = nullptr;
   int64_t tableStride = 0;
+  SeedRank seeds{};
   int* err = nullptr;
 };
 

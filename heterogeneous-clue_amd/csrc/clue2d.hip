@@ -75,12 +75,10 @@ void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DPara
   k_density2d<<<kCompPerSort * b.nBlocks, kThreads, 0, s>>>(b.blocks, keyS, b.xyw, perm, b.table, b.tableStride, prm.dc,
                                                  prm.dc2, b.xyrp, io.rho);
   int* parent = reinterpret_cast<int*>(b.k2);
-  int* rankAll = b.i2;
   k_nh2d<<<kCompPerSort * b.nBlocks, kThreads, 0, s>>>(b.blocks, keyS, b.xyrp, b.table, b.tableStride, b.evOff, prm.dc,
                                             prm.rhoc, prm.outlierDeltaFactor, prm.dm2, io.delta, io.nearestHigher,
                                             io.isSeed, parent);
-  scan_exclusive(io.isSeed, rankAll, b.N, b.part, s);
-  launch_labels(b.blocks, b.nBlocks, parent, io.isSeed, rankAll, b.evOff, io.clusterIndex, s);
+  launch_labels(b.blocks, b.nBlocks, parent, io.isSeed, b.N, b.evOff, io.clusterIndex, b.seeds, b.part, s);
 }
 
 // Carve a workspace of clue2d_workspace_bytes(N, E, nBlocks) bytes into the buffers of a batch.
@@ -104,6 +102,7 @@ void clue2d_carve(void* base, int64_t N, int E, int nBlocks, Clue2DBuffers& b) {
   b.tileStart = cv.take<int>((size_t)(table_tiles(k2dKeys) + 1) * (E > 0 ? E : 1));
   b.evOff = cv.take<int>((size_t)E + 1);
   b.blocks = cv.take<BlockDesc>((size_t)(nBlocks > 0 ? nBlocks : 1));
+  b.seeds = seed_rank_carve(cv, N);
 }
 
 size_t clue2d_workspace_bytes(int64_t N, int nEvents, int nBlocks) {
@@ -119,6 +118,7 @@ size_t clue2d_workspace_bytes(int64_t N, int nEvents, int nBlocks) {
   add((size_t)(table_tiles(k2dKeys) + 1) * (size_t)(nEvents > 0 ? nEvents : 1) * 4);  // tileStart
   add((size_t)(nEvents + 1) * 4);                                   // evOff
   add((size_t)(nBlocks > 0 ? nBlocks : 1) * sizeof(BlockDesc));     // blocks
+  bytes += seed_rank_bytes(N);                                      // seed words
   add(256);                                                         // error flags
   return bytes;
 }

@@ -43,6 +43,7 @@ struct Clue3DBuffers {
   int* table = nullptr;
   int* tileStart = nullptr;
   int64_t tableStride = 0;
+  SeedRank seeds{};
   int* err = nullptr;
 };
 

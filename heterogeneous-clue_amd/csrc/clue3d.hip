@@ -266,12 +266,10 @@ void clue3d_enqueue(const Clue3DBuffers& b, const Clue3DIO& io, const Clue3DPara
                                              io.radius, io.layer, io.isSilicon, b.table, b.tableStride, prm, b.rhoS,
                                              io.rho);
   int* parent = reinterpret_cast<int*>(b.k2);
-  int* rankAll = b.i2;
   k_nh3d<<<b.nBlocks, kThreads, 0, s>>>(b.blocks, perm, b.rozS, b.phiS, b.rhoS, io.z, io.r_over_absz, io.eta,
                                         io.radius, io.energy, io.layer, io.isSilicon, b.table, b.tableStride, b.evOff,
                                         prm, io.delta_f, io.delta_l, io.nh_layer, io.nh_idx, io.isSeed, parent);
-  scan_exclusive(io.isSeed, rankAll, b.N, b.part, s);
-  launch_labels(b.blocks, b.nBlocks, parent, io.isSeed, rankAll, b.evOff, io.tracksterIndex, s);
+  launch_labels(b.blocks, b.nBlocks, parent, io.isSeed, b.N, b.evOff, io.tracksterIndex, b.seeds, b.part, s);
 }
 
 // Carve a workspace of clue3d_workspace_bytes(N, E, nBlocks) bytes into the buffers of a batch.
@@ -297,6 +295,7 @@ void clue3d_carve(void* base, int64_t N, int E, int nBlocks, Clue3DBuffers& b) {
   b.tileStart = cv.take<int>((size_t)(table_tiles(k3dKeys) + 1) * (E > 0 ? E : 1));
   b.evOff = cv.take<int>((size_t)E + 1);
   b.blocks = cv.take<BlockDesc>((size_t)(nBlocks > 0 ? nBlocks : 1));
+  b.seeds = seed_rank_carve(cv, N);
 }
 
 size_t clue3d_workspace_bytes(int64_t N, int nEvents, int nBlocks) {
@@ -311,6 +310,7 @@ size_t clue3d_workspace_bytes(int64_t N, int nEvents, int nBlocks) {
   add((size_t)(table_tiles(k3dKeys) + 1) * (size_t)(nEvents > 0 ? nEvents : 1) * 4);  // tileStart
   add((size_t)(nEvents + 1) * 4);
   add((size_t)(nBlocks > 0 ? nBlocks : 1) * sizeof(BlockDesc));
+  bytes += seed_rank_bytes(N);  // seed words
   add(256);
   return bytes;
 }

@@ -202,13 +202,6 @@ inline int table_tiles(int nKeys) { return (nKeys + 1 + kTableTile - 1) / kTable
 void build_table(const uint32_t* keySorted, const BlockDesc* blocksDev, int nBlocks, const int* evOffDev,
                  int nEvents, int nKeys, int64_t stride, int* table, int* tileStart, hipStream_t s);
 
-// ---------------------------------------------------------------- seed labels
-// clusterIndex[i] = isSeed[root(i)] ? (#seeds of the event before root) : -1, where root(i) is
-// reached by following parent[] (parent[i] == i at roots). rankAll is the exclusive scan of
-// isSeed over the batch. parent[] is shortened in place while walking.
-void launch_labels(const BlockDesc* blocksDev, int nBlocks, int* parent, const int* isSeed, const int* rankAll,
-                   const int* evOffDev, int32_t* label, hipStream_t s);
-
 // Sequential carving of one allocation into 256-byte aligned sub-buffers.
 struct Carver {
   char* p;
@@ -219,6 +212,38 @@ struct Carver {
     return r;
   }
 };
+
+// ---------------------------------------------------------------- seed labels
+// Seed ranks in point-index order without a scan of the whole isSeed array: one pass packs isSeed
+// into 64-bit words (one per 64 points) with the exclusive prefix of each word inside its tile of
+// kSeedTile points (u16) and the tile's count; a scan of the tile counts completes the ranks:
+//   rank(i) = tilePre[i / kSeedTile] + wordPre[i / 64] + popcount(word[i / 64] below bit i % 64).
+constexpr int kSeedTile = kScanTile;  // 2048 points, 32 words
+struct SeedRank {
+  uint64_t* word;
+  uint16_t* wordPre;
+  int* tilePre;
+};
+inline int seed_tiles(int64_t N) { return (int)((N + kSeedTile - 1) / kSeedTile); }
+inline size_t seed_rank_bytes(int64_t N) {
+  const size_t t = (size_t)(seed_tiles(N) > 0 ? seed_tiles(N) : 1);
+  auto a = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  return a(t * 32 * sizeof(uint64_t)) + a(t * 32 * sizeof(uint16_t)) + a((t + 1) * sizeof(int));
+}
+inline SeedRank seed_rank_carve(Carver& cv, int64_t N) {
+  const size_t t = (size_t)(seed_tiles(N) > 0 ? seed_tiles(N) : 1);
+  SeedRank r;
+  r.word = cv.take<uint64_t>(t * 32);
+  r.wordPre = cv.take<uint16_t>(t * 32);
+  r.tilePre = cv.take<int>(t + 1);
+  return r;
+}
+
+// clusterIndex[i] = isSeed[root(i)] ? (#seeds of the event before root) : -1, where root(i) is
+// reached by following parent[] (parent[i] == i at roots). parent[] is shortened in place while
+// walking. `part` needs scan_partials(seed_tiles(N)) ints.
+void launch_labels(const BlockDesc* blocksDev, int nBlocks, int* parent, const int* isSeed, int64_t N,
+                   const int* evOffDev, int32_t* label, const SeedRank& sr, int* part, hipStream_t s);
 
 // Fill an int buffer on the stream with a kernel (graph-capture friendly, no memset nodes).
 void fill_i32(int* p, int value, int64_t n, hipStream_t s);

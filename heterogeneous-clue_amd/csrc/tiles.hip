@@ -39,10 +39,13 @@ void build_table(const uint32_t* keySorted, const BlockDesc* blocksDev, int nBlo
   k_table<<<nEvents * tiles, kThreads, 0, s>>>(keySorted, evOffDev, tileStart, tiles, nKeys, stride, table);
 }
 
-void launch_labels(const BlockDesc* blocksDev, int nBlocks, int* parent, const int* isSeed, const int* rankAll,
-                   const int* evOffDev, int32_t* label, hipStream_t s) {
-  if (nBlocks <= 0) return;
-  k_labels<<<nBlocks, kThreads, 0, s>>>(blocksDev, parent, isSeed, rankAll, evOffDev, label);
+void launch_labels(const BlockDesc* blocksDev, int nBlocks, int* parent, const int* isSeed, int64_t N,
+                   const int* evOffDev, int32_t* label, const SeedRank& sr, int* part, hipStream_t s) {
+  if (nBlocks <= 0 || N <= 0) return;
+  const int tiles = seed_tiles(N);
+  k_seed_bits<<<tiles, kThreads, 0, s>>>(isSeed, N, sr);
+  scan_exclusive(sr.tilePre, sr.tilePre, tiles, part, s);
+  k_labels<<<nBlocks, kThreads, 0, s>>>(blocksDev, parent, sr, evOffDev, label);
 }
 
 void fill_i32(int* p, int value, int64_t n, hipStream_t s) {

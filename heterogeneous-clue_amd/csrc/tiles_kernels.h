@@ -144,12 +144,43 @@ __global__ __launch_bounds__(kThreads) void k_table(const uint32_t* __restrict__
 
 // ---------------------------------------------------------------- labels
 
+// isSeed -> seed words, in-tile word prefixes and tile counts (tilePre, scanned afterwards).
+__global__ __launch_bounds__(kThreads) void k_seed_bits(const int* __restrict__ isSeed, int64_t n,
+                                                        SeedRank sr) {
+  __shared__ int s_cnt[kSeedTile / kWave];
+  const int tid = (int)threadIdx.x, lane = lane_id(), wave = tid / kWave;
+  const int64_t base = (int64_t)blockIdx.x * kSeedTile;
+  const int64_t w0 = (int64_t)blockIdx.x * (kSeedTile / kWave);
+#pragma unroll
+  for (int k = 0; k < kSeedTile / kThreads; ++k) {
+    const int64_t i = base + k * kThreads + tid;
+    const uint64_t m = __ballot(i < n && isSeed[i] != 0);
+    if (lane == 0) {
+      sr.word[w0 + k * kWaves + wave] = m;
+      s_cnt[k * kWaves + wave] = __popcll(m);
+    }
+  }
+  __syncthreads();
+  if (tid < kWave) {
+    const int v = tid < kSeedTile / kWave ? s_cnt[tid] : 0;
+    const int inc = wave_inclusive_sum(v);
+    if (tid < kSeedTile / kWave) sr.wordPre[w0 + tid] = (uint16_t)(inc - v);
+    if (tid == kSeedTile / kWave - 1) sr.tilePre[blockIdx.x] = inc;
+  }
+}
+
+__device__ __forceinline__ bool seed_of(const SeedRank& sr, int i) {
+  return (sr.word[i >> 6] >> (i & 63)) & 1ull;
+}
+__device__ __forceinline__ int rank_of(const SeedRank& sr, int i) {
+  return sr.tilePre[i / kSeedTile] + sr.wordPre[i >> 6] + __popcll(sr.word[i >> 6] & ((1ull << (i & 63)) - 1ull));
+}
+
 __global__ __launch_bounds__(kThreads) void k_labels(const BlockDesc* __restrict__ blocks, int* parent,
-                                                     const int* __restrict__ isSeed, const int* __restrict__ rankAll,
-                                                     const int* __restrict__ evOff, int32_t* __restrict__ label) {
+                                                     SeedRank sr, const int* __restrict__ evOff,
+                                                     int32_t* __restrict__ label) {
   const BlockDesc bd = blocks[xcd_block((int)blockIdx.x, (int)gridDim.x)];
-  const int evBase = evOff[bd.event];
-  const int rank0 = rankAll[evBase];
+  const int rank0 = rank_of(sr, evOff[bd.event]);
 #pragma unroll 1
   for (int r = 0; r < kSortItems; ++r) {
     const int i = bd.begin + r * kThreads + (int)threadIdx.x;
@@ -163,10 +194,9 @@ __global__ __launch_bounds__(kThreads) void k_labels(const BlockDesc* __restrict
       cur = nxt;
     }
     parent[i] = cur;
-    label[i] = isSeed[cur] ? rankAll[cur] - rank0 : -1;
+    label[i] = seed_of(sr, cur) ? rank_of(sr, cur) - rank0 : -1;
   }
 }
-
 
 __global__ void k_fill(int* p, int v, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -60,13 +60,12 @@ int main(int argc, char** argv) {
                   rho.data());
     });
     int* parent = reinterpret_cast<int*>(k2.data());
-    int* rankAll = i2.data();
     emu::launch(kCompPerSort * nb, kThreads, [&] {
       k_nh2d(blocks.data(), k1.data(), xyrp.data(), table.data(), stride, evOff.data(), dc, rhoc, odf, dm2,
              delta.data(), nh.data(), isSeed.data(), parent);
     });
-    scan_exclusive(isSeed.data(), rankAll, N, part.data());
-    launch_labels(blocks.data(), nb, parent, isSeed.data(), rankAll, evOff.data(), ci.data());
+    SeedRankHost seeds(N);
+    launch_labels(blocks.data(), nb, parent, isSeed.data(), N, evOff.data(), ci.data(), seeds.view(), part.data());
   }
   write_output(argv[2], err, 0, rho, delta, nh, isSeed, ci);
   return 0;

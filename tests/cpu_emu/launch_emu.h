@@ -30,11 +30,28 @@ inline void build_table(const uint32_t* key, const BlockDesc* blocks, int nBlock
   emu::launch(nEvents * tiles, kThreads, [&] { k_table(key, evOff, tileStart, tiles, nKeys, stride, table); });
 }
 
-inline void launch_labels(const BlockDesc* blocks, int nBlocks, int* parent, const int* isSeed, const int* rankAll,
-                          const int* evOff, int32_t* label) {
-  if (nBlocks <= 0) return;
-  emu::launch(nBlocks, kThreads, [&] { k_labels(blocks, parent, isSeed, rankAll, evOff, label); });
+inline void launch_labels(const BlockDesc* blocks, int nBlocks, int* parent, const int* isSeed, int64_t N,
+                          const int* evOff, int32_t* label, const SeedRank& sr, int* part) {
+  if (nBlocks <= 0 || N <= 0) return;
+  const int tiles = seed_tiles(N);
+  emu::launch(tiles, kThreads, [&] { k_seed_bits(isSeed, N, sr); });
+  scan_exclusive(sr.tilePre, sr.tilePre, tiles, part);
+  emu::launch(nBlocks, kThreads, [&] { k_labels(blocks, parent, sr, evOff, label); });
 }
+
+// Seed-rank buffers of N points, poisoned like fresh device memory.
+struct SeedRankHost {
+  std::vector<uint64_t> word;
+  std::vector<uint16_t> wordPre;
+  std::vector<int> tilePre;
+  explicit SeedRankHost(int64_t N) {
+    const size_t t = (size_t)std::max(seed_tiles(N), 1);
+    word.assign(t * 32, 0x5A5A5A5A5A5A5A5Aull);
+    wordPre.assign(t * 32, 0x5A5A);
+    tilePre.assign(t + 1, 0x5A5A5A5A);
+  }
+  SeedRank view() { return SeedRank{word.data(), wordPre.data(), tilePre.data()}; }
+};
 
 // Block table of a packed batch, as make_blocks builds it.
 inline std::vector<BlockDesc> blocks_of(const std::vector<int64_t>& offs) {
