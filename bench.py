@@ -34,6 +34,21 @@ BYTES_PER_HIT_2D = 36  # in: x, y, weight f32 + layer i32; out: rho, delta f32 +
 BYTES_PER_CLUSTER_3D = 56
 
 
+TRAFFIC_PROFILE = REPO / "profiles" / "clue2d_batched_256ev_hbm_traffic.md"
+
+
+def measured_traffic():
+    """HBM bytes per graph replay of the 256-event batch, from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (scaled by the copy-probe calibration recorded there), or None when absent."""
+    try:
+        for line in TRAFFIC_PROFILE.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -212,10 +227,15 @@ def main():
     }
     algo_bytes = BYTES_PER_HIT_2D * N
     achieved = algo_bytes / (ms * 1e-3) / 1e9
+    traffic = measured_traffic() if args.events == 256 else None
     result["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                           "kernel": "batched CLUE2D launch (all kernels of one graph replay)",
                           "algorithmic_bytes_per_launch": algo_bytes}
+    if traffic:
+        result["roofline"]["traffic_source"] = str(TRAFFIC_PROFILE.relative_to(REPO))
+        result["roofline"]["traffic_over_algorithmic"] = round(traffic / algo_bytes, 2)
+        result["roofline"]["hbm_gbs_measured_traffic"] = round(traffic / (ms * 1e-3) / 1e9, 1)
 
     if not args.no_per_event:
         pout = dev.alloc_soa(N, clue_hip.FIELDS_2D_OUT)
