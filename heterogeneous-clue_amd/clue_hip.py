@@ -17,6 +17,7 @@ CLUE_OK = 0
 ERRORS = {-1: "CLUE_ERR_ARG", -2: "CLUE_ERR_HIP", -3: "CLUE_ERR_BUDGET", -4: "CLUE_ERR_LAYER", -5: "CLUE_ERR_BIN",
           -6: "CLUE_ERR_TIMEOUT", -7: "CLUE_ERR_CAPACITY"}
 PATH_AUTO, PATH_FUSED, PATH_GLOBAL = 0, 1, 2
+PLAN_GRAPH, PLAN_AUTOTUNE = 1, 2
 FUSED_CAPACITY = 10112
 
 # Every symbol include/clue_hip.h declares.
@@ -30,7 +31,7 @@ EXPORTS = [
     "clue3d_run_batch_device", "clue3d_plan_create", "clue_plan_launch", "clue_plan_destroy",
     "clue_plan_workspace_bytes", "clue_selftest_arith", "clue_server_create", "clue_server_destroy",
     "clue_server_submit2d", "clue_server_submit3d", "clue_server_wait", "clue_server_stats",
-    "clue_server_last_error", "clue_set_path", "clue_plan_path",
+    "clue_server_last_error", "clue_set_path", "clue_plan_path", "clue_plan_tuning",
 ]
 
 
@@ -109,6 +110,7 @@ def load_library(path: Path = LIB_PATH) -> C.CDLL:
         "clue_server_last_error": (C.c_char_p, [VP]),
         "clue_set_path": (I32, [VP, I32, I32]),
         "clue_plan_path": (I32, [VP]),
+        "clue_plan_tuning": (I32, [VP, VP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -247,12 +249,13 @@ class Device:
                                                    *[dout[k].ptr for k in FIELDS_2D_OUT], stream))
 
     def clue2d_plan(self, din: dict, dout: dict, offs: np.ndarray, p: Clue2DParams | None = None,
-                    use_graph: bool = True) -> "Plan":
+                    use_graph: bool = True, autotune: bool = False) -> "Plan":
         p = p or params2d()
         offs = np.ascontiguousarray(offs, dtype=np.int64)
+        flags = (PLAN_GRAPH if use_graph else 0) | (PLAN_AUTOTUNE if autotune else 0)
         h = self.lib.clue2d_plan_create(self.ctx, C.byref(p), offs.size - 1, offs.ctypes.data,
                                         *[din[k].ptr for k in FIELDS_2D_IN], *[dout[k].ptr for k in FIELDS_2D_OUT],
-                                        1 if use_graph else 0)
+                                        flags)
         if not h:
             raise ClueError(-1, self.error())
         return Plan(self, h)
@@ -376,6 +379,14 @@ class Plan:
     @property
     def workspace_bytes(self) -> int:
         return int(self.dev.lib.clue_plan_workspace_bytes(self.h))
+
+    @property
+    def tuning(self) -> dict:
+        """Points per workgroup and staged points of the density and nearest-higher kernels."""
+        out = np.zeros(4, np.int32)
+        self.dev._chk(self.dev.lib.clue_plan_tuning(self.h, out.ctypes.data))
+        return {"density_points": int(out[0]), "density_stage": int(out[1]), "nh_points": int(out[2]),
+                "nh_stage": int(out[3])}
 
     @property
     def path(self) -> int:

@@ -33,6 +33,7 @@ struct clue_plan {
   Clue3DIO io3{};
   Clue3DParams p3{};
   bool useGraph = false;
+  bool tune = false;   // CLUE_PLAN_AUTOTUNE requested and not done yet
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -219,6 +220,44 @@ int plan_route_first(clue_plan* pl, hipStream_t s) {
     clue2d_enqueue(pl->b2, pl->io2, pl->p2, s);
   }
   HIP_TRY(c, hipGetLastError());
+  return CLUE_OK;
+}
+
+// CLUE2D plan autotuning: one full run (so this launch produces its results), then each variant of
+// the density and of the nearest-higher kernel timed in place (best of 3) on the same data; the
+// fastest of each is kept. Re-running either kernel rewrites its outputs with the same values.
+int plan_autotune(clue_plan* pl, hipStream_t s) {
+  clue_ctx* c = pl->ctx;
+  clue2d_enqueue(pl->b2, pl->io2, pl->p2, s);
+  HIP_TRY(c, hipGetLastError());
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(c, hipEventCreate(&e0));
+  HIP_TRY(c, hipEventCreate(&e1));
+  int best[2] = {0, 0};
+  for (int which = 0; which < 2; ++which) {
+    float bestMs = 0.f;
+    for (int v = 0; v < kClue2DVariants; ++v) {
+      float ms = 0.f;
+      for (int rep = 0; rep < 4; ++rep) {  // the first run of each variant warms it up
+        HIP_TRY(c, hipEventRecord(e0, s));
+        clue2d_enqueue_step(pl->b2, pl->io2, pl->p2, which, v, s);
+        HIP_TRY(c, hipEventRecord(e1, s));
+        HIP_TRY(c, hipEventSynchronize(e1));
+        float t = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&t, e0, e1));
+        if (rep == 1 || (rep > 1 && t < ms)) ms = t;
+      }
+      if (v == 0 || ms < bestMs) {
+        bestMs = ms;
+        best[which] = v;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  pl->b2.densityVariant = best[0];
+  pl->b2.nhVariant = best[1];
+  pl->tune = false;
   return CLUE_OK;
 }
 
@@ -571,7 +610,8 @@ clue_plan* clue2d_plan_create(clue_ctx* c, const clue2d_params* p, int32_t E, co
   pl->offsets.assign(offsets, offsets + E + 1);
   pl->io2 = Clue2DIO{x, y, layer, weight, rho, delta, nh, isSeed, clusterIndex};
   pl->p2 = to2d(p);
-  pl->useGraph = use_graph != 0;
+  pl->useGraph = (use_graph & CLUE_PLAN_GRAPH) != 0;
+  pl->tune = (use_graph & CLUE_PLAN_AUTOTUNE) != 0;
   if (c->path2d == CLUE_PATH_GLOBAL) {
     pl->route2d = CLUE_PATH_GLOBAL;
     if (plan_setup_global(pl) != CLUE_OK) {
@@ -712,7 +752,7 @@ clue_plan* clue3d_plan_create(clue_ctx* c, const clue3d_params* p, int32_t E, co
   }
   pl->io3 = Clue3DIO{z, eta, phi, roz, radius, energy, layer, isSi, rho, df, dl, nhl, nhi, isSeed, ti};
   pl->p3 = to3d(p);
-  pl->useGraph = use_graph != 0;
+  pl->useGraph = (use_graph & CLUE_PLAN_GRAPH) != 0;
   return pl;
 }
 
@@ -723,6 +763,7 @@ int32_t clue_plan_launch(clue_plan* pl, void* stream) {
   clue_ctx* c = pl->ctx;
   hipStream_t s = pick(c, stream);
   if (pl->dim == 2 && pl->route2d < 0) return plan_route_first(pl, s);
+  if (pl->dim == 2 && pl->route2d == CLUE_PATH_GLOBAL && pl->tune) return plan_autotune(pl, s);
   if (!pl->useGraph) return launch_plan_body(pl, s);
   if (!pl->exec) {
     // Record the launch sequence once on a private stream, then replay it.
@@ -756,6 +797,18 @@ void clue_plan_destroy(clue_plan* pl) {
 int32_t clue_plan_path(const clue_plan* pl) {
   if (!pl) return CLUE_ERR_ARG;
   return pl->dim == 2 ? pl->route2d : CLUE_PATH_GLOBAL;
+}
+
+int32_t clue_plan_tuning(const clue_plan* pl, int32_t* out4) {
+  if (!pl || !out4) return CLUE_ERR_ARG;
+  int di = 0, ds = 0, ni = 0, ns = 0;
+  clue2d_variant_shape(pl->b2.densityVariant, di, ds);
+  clue2d_variant_shape(pl->b2.nhVariant, ni, ns);
+  out4[0] = di * kThreads;
+  out4[1] = ds;
+  out4[2] = ni * kThreads;
+  out4[3] = ns;
+  return CLUE_OK;
 }
 
 uint64_t clue_plan_workspace_bytes(const clue_plan* pl) { return pl ? pl->ws.bytes + pl->wsF.bytes : 0; }

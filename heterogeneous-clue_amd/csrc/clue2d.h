@@ -43,6 +43,8 @@ struct Clue2DBuffers {
   int64_t tableStride = 0;
   SeedRank seeds{};
   int* err = nullptr;
+  int densityVariant = 0;  // (points per thread, staging) variant of the density kernel (kVariants2D)
+  int nhVariant = 0;       // the same for the nearest-higher kernel
 };
 
 // Caller-owned device inputs/outputs, packed over the batch (nearestHigher and clusterIndex are
@@ -62,6 +64,14 @@ struct Clue2DIO {
 constexpr int64_t clue2d_table_stride() { return ((int64_t)k2dKeys + 1 + 63) / 64 * 64; }
 
 void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DParams& p, hipStream_t s);
+// The density (which = 0) or nearest-higher (which = 1) kernel alone with the given variant, on a
+// batch clue2d_enqueue has run on: both read only what earlier steps left and rewrite their own
+// outputs with the same values, so the autotuner can time them in place.
+void clue2d_enqueue_step(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DParams& p, int which, int variant,
+                         hipStream_t s);
+constexpr int kClue2DVariants = 4;
+// Points per thread and staged points of a variant.
+void clue2d_variant_shape(int variant, int& items, int& stage);
 size_t clue2d_workspace_bytes(int64_t N, int nEvents, int nBlocks);
 void clue2d_carve(void* base, int64_t N, int E, int nBlocks, Clue2DBuffers& b);
 

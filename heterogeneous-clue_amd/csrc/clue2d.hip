@@ -49,6 +49,41 @@ Clue2DParams make_clue2d_params(float dc, float rhoc, float odf) {
 
 using namespace g2d;
 
+namespace {
+
+template <int V, class... A>
+void density_v(int nBlocks, hipStream_t s, A... a) {
+  constexpr Variant2D v = kVariants2D[V];
+  k_density2d<v.items, v.stage><<<CompShape<v.items>::kPerSort * nBlocks, kThreads, 0, s>>>(a...);
+}
+template <int V, class... A>
+void nh_v(int nBlocks, hipStream_t s, A... a) {
+  constexpr Variant2D v = kVariants2D[V];
+  k_nh2d<v.items, v.stage><<<CompShape<v.items>::kPerSort * nBlocks, kThreads, 0, s>>>(a...);
+}
+
+}  // namespace
+
+template <class... A>
+void launch_density2d(int variant, int nBlocks, hipStream_t s, A... a) {
+  switch (variant) {
+    case 1: density_v<1>(nBlocks, s, a...); break;
+    case 2: density_v<2>(nBlocks, s, a...); break;
+    case 3: density_v<3>(nBlocks, s, a...); break;
+    default: density_v<0>(nBlocks, s, a...); break;
+  }
+}
+template <class... A>
+void launch_nh2d(int variant, int nBlocks, hipStream_t s, A... a) {
+  switch (variant) {
+    case 1: nh_v<1>(nBlocks, s, a...); break;
+    case 2: nh_v<2>(nBlocks, s, a...); break;
+    case 3: nh_v<3>(nBlocks, s, a...); break;
+    default: nh_v<0>(nBlocks, s, a...); break;
+  }
+}
+static_assert(kNumVariants2D == 4, "launch_density2d / launch_nh2d dispatch four variants");
+
 void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DParams& prm, hipStream_t s) {
   if (b.N <= 0 || b.nBlocks <= 0) return;
   const int nHist = kRadix * b.nBlocks;
@@ -72,13 +107,32 @@ void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DPara
   const uint32_t* keyS = b.k1;
   const int* perm = b.i1;
   build_table(keyS, b.blocks, b.nBlocks, b.evOff, b.E, k2dKeys, b.tableStride, b.table, b.tileStart, s);
-  k_density2d<<<kCompPerSort * b.nBlocks, kThreads, 0, s>>>(b.blocks, keyS, b.xyw, perm, b.table, b.tableStride, prm.dc,
-                                                 prm.dc2, b.xyrp, io.rho);
+  launch_density2d(b.densityVariant, b.nBlocks, s, b.blocks, keyS, b.xyw, perm, b.table, b.tableStride, prm.dc,
+                   prm.dc2, b.xyrp, io.rho);
   int* parent = reinterpret_cast<int*>(b.k2);
-  k_nh2d<<<kCompPerSort * b.nBlocks, kThreads, 0, s>>>(b.blocks, keyS, b.xyrp, b.table, b.tableStride, b.evOff, prm.dc,
-                                            prm.rhoc, prm.outlierDeltaFactor, prm.dm2, io.delta, io.nearestHigher,
-                                            io.isSeed, parent);
+  launch_nh2d(b.nhVariant, b.nBlocks, s, b.blocks, keyS, b.xyrp, b.table, b.tableStride, b.evOff, prm.dc, prm.rhoc,
+              prm.outlierDeltaFactor, prm.dm2, io.delta, io.nearestHigher, io.isSeed, parent);
   launch_labels(b.blocks, b.nBlocks, parent, io.isSeed, b.N, b.evOff, io.clusterIndex, b.seeds, b.part, s);
+}
+
+void clue2d_enqueue_step(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DParams& prm, int which, int variant,
+                         hipStream_t s) {
+  if (b.N <= 0 || b.nBlocks <= 0) return;
+  const uint32_t* keyS = b.k1;
+  const int* perm = b.i1;
+  int* parent = reinterpret_cast<int*>(b.k2);
+  if (which == 0)
+    launch_density2d(variant, b.nBlocks, s, b.blocks, keyS, b.xyw, perm, b.table, b.tableStride, prm.dc, prm.dc2,
+                     b.xyrp, io.rho);
+  else
+    launch_nh2d(variant, b.nBlocks, s, b.blocks, keyS, b.xyrp, b.table, b.tableStride, b.evOff, prm.dc, prm.rhoc,
+                prm.outlierDeltaFactor, prm.dm2, io.delta, io.nearestHigher, io.isSeed, parent);
+}
+
+void clue2d_variant_shape(int variant, int& items, int& stage) {
+  const Variant2D v = kVariants2D[variant >= 0 && variant < kNumVariants2D ? variant : 0];
+  items = v.items;
+  stage = v.stage;
 }
 
 // Carve a workspace of clue2d_workspace_bytes(N, E, nBlocks) bytes into the buffers of a batch.

@@ -67,20 +67,27 @@ using DigitY = Digit<1, k2dRows>;
 using DigitX = Digit<k2dRows, k2dCols>;
 using DigitL = Digit<k2dBinsPerLayer, kRadix>;
 
-// The density and nearest-higher kernels give each block kCompHits consecutive sorted points of one
-// event. In x-major tile order everything their search boxes can reach is one contiguous range of
-// the sorted array: from the first bin of the lowest box to the end of the highest box. When that
-// range has at most kStage points it is staged into LDS once and every candidate is read from
-// there; otherwise candidates are read from global memory. Either way the candidates are visited
-// in the serial order, so the results do not depend on which path ran.
-constexpr int kCompItems = 2;
-constexpr int kCompHits = kThreads * kCompItems;    // points per compute block
-constexpr int kCompPerSort = kSortBlock / kCompHits;  // compute blocks per sort block
-constexpr int kStage = 1024;                         // staged points (16 KB of float4)
-constexpr int kMaxCols = 5;                          // column runs kept in registers per point
+// The density and nearest-higher kernels give each block ITEMS * kThreads consecutive sorted points
+// of one event (ITEMS points per thread). In x-major tile order everything their search boxes can
+// reach is one contiguous range of the sorted array: from the first bin of the lowest box to the end
+// of the highest box. When that range has at most STAGE points it is staged into LDS once and every
+// candidate is read from there; otherwise candidates are read from global memory. Either way the
+// candidates are visited in the serial order, so the results do not depend on ITEMS, STAGE or which
+// path ran; the plan autotuner picks (ITEMS, STAGE) per kernel among kVariants2D by timing.
+template <int ITEMS>
+struct CompShape {
+  static constexpr int kHits = kThreads * ITEMS;         // points per compute block
+  static constexpr int kPerSort = kSortBlock / kHits;    // compute blocks per sort block
+  static_assert(kSortBlock % kHits == 0, "compute blocks tile each sort block");
+};
+constexpr int kMaxCols = 5;  // column runs kept in registers per point
 
-static_assert(kSortBlock % kCompHits == 0, "compute blocks tile each sort block");
-static_assert(kStage % kThreads == 0, "staging splits evenly over the block");
+struct Variant2D {
+  int items;  // points per thread
+  int stage;  // staged points
+};
+constexpr int kNumVariants2D = 4;
+constexpr Variant2D kVariants2D[kNumVariants2D] = {{2, 1024}, {1, 1024}, {2, 2048}, {4, 2048}};
 
 // A search box of the reference (LayerTilesSerial::searchBox) as column runs: run k covers column
 // xlo + k, rows ylo..yhi, which are the sorted positions [s[k], t[k]). Columns beyond kMaxCols
@@ -137,10 +144,12 @@ __device__ __forceinline__ void block_range(int lo, int hi, int* s_red, int& out
   }
 }
 
-// Copy [lo, hi) (1 <= hi - lo <= kStage points) into LDS, all loads of a thread in flight together
+// Copy [lo, hi) (1 <= hi - lo <= STAGE points) into LDS, all loads of a thread in flight together
 // (indices past the end re-read the last point and are not stored).
+template <int STAGE>
 __device__ __forceinline__ void stage_range(const float4* __restrict__ g, float4* st, int lo, int hi) {
-  constexpr int kPer = kStage / kThreads;
+  static_assert(STAGE % kThreads == 0, "staging splits evenly over the block");
+  constexpr int kPer = STAGE / kThreads;
   const int n = hi - lo;
   float4 v[kPer];
 #pragma unroll
@@ -251,24 +260,26 @@ __device__ __forceinline__ float density_run(int p, float xi, float yi, int s, i
   return rho;
 }
 
+template <int ITEMS, int STAGE>
 __global__ __launch_bounds__(kThreads) void k_density2d(const BlockDesc* __restrict__ blocks,
                                                         const uint32_t* __restrict__ keyS,
                                                         const float4* __restrict__ xyw, const int* __restrict__ perm,
                                                         const int* __restrict__ table, int64_t stride, float dc,
                                                         float dc2, float4* __restrict__ xyrp,
                                                         float* __restrict__ rhoOut) {
-  __shared__ float4 s_st[kStage];
+  using Shape = CompShape<ITEMS>;
+  __shared__ float4 s_st[STAGE];
   __shared__ int s_red[2 * kWaves];
-  const BlockDesc bd = blocks[blockIdx.x / kCompPerSort];
-  const int b0 = bd.begin + (int)(blockIdx.x % kCompPerSort) * kCompHits;
-  const int b1 = (b0 + kCompHits) < bd.end ? (b0 + kCompHits) : bd.end;
+  const BlockDesc bd = blocks[blockIdx.x / Shape::kPerSort];
+  const int b0 = bd.begin + (int)(blockIdx.x % Shape::kPerSort) * Shape::kHits;
+  const int b1 = (b0 + Shape::kHits) < bd.end ? (b0 + Shape::kHits) : bd.end;
   if (b0 >= b1) return;  // uniform over the block
   const int* T = table + (int64_t)bd.event * stride;
-  Runs runs[kCompItems];
-  float4 me[kCompItems];
+  Runs runs[ITEMS];
+  float4 me[ITEMS];
   int lo = 0x7fffffff, hi = -1;
 #pragma unroll
-  for (int r = 0; r < kCompItems; ++r) {
+  for (int r = 0; r < ITEMS; ++r) {
     const int p = b0 + r * kThreads + (int)threadIdx.x;
     if (p < b1) {
       me[r] = xyw[p];
@@ -280,11 +291,11 @@ __global__ __launch_bounds__(kThreads) void k_density2d(const BlockDesc* __restr
   }
   int rlo, rhi;
   block_range(lo, hi, s_red, rlo, rhi);
-  const bool staged = rhi - rlo <= kStage;
-  if (staged) stage_range(xyw, s_st, rlo, rhi);
+  const bool staged = rhi - rlo <= STAGE;
+  if (staged) stage_range<STAGE>(xyw, s_st, rlo, rhi);
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < kCompItems; ++r) {
+  for (int r = 0; r < ITEMS; ++r) {
     const int p = b0 + r * kThreads + (int)threadIdx.x;
     if (p < b1) {
       const float xi = me[r].x, yi = me[r].y;
@@ -420,6 +431,7 @@ __device__ __forceinline__ int nearest_higher_of(int i, float xi, float yi, floa
 
 // delta / nearestHigher plus the seed/outlier decision of kernel_findAndAssign_clusters
 // (CLUEAlgoKernels.h:108-137).
+template <int ITEMS, int STAGE>
 __global__ __launch_bounds__(kThreads) void k_nh2d(const BlockDesc* __restrict__ blocks,
                                                    const uint32_t* __restrict__ keyS,
                                                    const float4* __restrict__ xyrp, const int* __restrict__ table,
@@ -427,20 +439,21 @@ __global__ __launch_bounds__(kThreads) void k_nh2d(const BlockDesc* __restrict__
                                                    float odf, float dm2, float* __restrict__ deltaOut,
                                                    int32_t* __restrict__ nhOut, int32_t* __restrict__ isSeedOut,
                                                    int* __restrict__ parent) {
-  __shared__ float4 s_st[kStage];
+  using Shape = CompShape<ITEMS>;
+  __shared__ float4 s_st[STAGE];
   __shared__ int s_red[2 * kWaves];
-  const BlockDesc bd = blocks[blockIdx.x / kCompPerSort];
-  const int b0 = bd.begin + (int)(blockIdx.x % kCompPerSort) * kCompHits;
-  const int b1 = (b0 + kCompHits) < bd.end ? (b0 + kCompHits) : bd.end;
+  const BlockDesc bd = blocks[blockIdx.x / Shape::kPerSort];
+  const int b0 = bd.begin + (int)(blockIdx.x % Shape::kPerSort) * Shape::kHits;
+  const int b1 = (b0 + Shape::kHits) < bd.end ? (b0 + Shape::kHits) : bd.end;
   if (b0 >= b1) return;
   const int* T = table + (int64_t)bd.event * stride;
   const int evBase = evOff[bd.event];
   const float dm = odf * dc;
-  Runs runs[kCompItems];
-  float4 me[kCompItems];
+  Runs runs[ITEMS];
+  float4 me[ITEMS];
   int lo = 0x7fffffff, hi = -1;
 #pragma unroll
-  for (int r = 0; r < kCompItems; ++r) {
+  for (int r = 0; r < ITEMS; ++r) {
     const int p = b0 + r * kThreads + (int)threadIdx.x;
     if (p < b1) {
       me[r] = xyrp[p];
@@ -452,11 +465,11 @@ __global__ __launch_bounds__(kThreads) void k_nh2d(const BlockDesc* __restrict__
   }
   int rlo, rhi;
   block_range(lo, hi, s_red, rlo, rhi);
-  const bool staged = rhi - rlo <= kStage;
-  if (staged) stage_range(xyrp, s_st, rlo, rhi);
+  const bool staged = rhi - rlo <= STAGE;
+  if (staged) stage_range<STAGE>(xyrp, s_st, rlo, rhi);
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < kCompItems; ++r) {
+  for (int r = 0; r < ITEMS; ++r) {
     const int p = b0 + r * kThreads + (int)threadIdx.x;
     if (p < b1) {
       const int i = __float_as_int(me[r].w);

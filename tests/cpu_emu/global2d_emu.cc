@@ -1,12 +1,31 @@
 // Runs the CLUE2D global-sort path (three radix passes, bin table, density, nearest higher, seed
 // ranks, labels) on the CPU through the HIP-model emulation, with clue2d_enqueue's launch sequence
-// and grids. Test infrastructure only. usage: global2d_emu <in> <out> (formats: launch_emu.h)
+// and grids. Test infrastructure only.
+// usage: global2d_emu <in> <out> [density variant] [nearest-higher variant] (formats: launch_emu.h;
+// variants index kVariants2D)
+#include <type_traits>
+
 #include "clue2d_kernels.h"
 #include "launch_emu.h"
 
 using namespace clue;
 using namespace clue::g2d;
 using namespace emu_host;
+
+template <int V, class F>
+void with_variant(F&& f) {
+  constexpr Variant2D v = kVariants2D[V];
+  f(std::integral_constant<int, v.items>{}, std::integral_constant<int, v.stage>{});
+}
+template <class F>
+void dispatch(int variant, F&& f) {
+  switch (variant) {
+    case 1: with_variant<1>(f); break;
+    case 2: with_variant<2>(f); break;
+    case 3: with_variant<3>(f); break;
+    default: with_variant<0>(f); break;
+  }
+}
 
 int main(int argc, char** argv) {
   if (argc < 3) return 2;
@@ -55,14 +74,19 @@ int main(int argc, char** argv) {
           EmitLast2D{i2.data(), k1.data(), i1.data(), in.x.data(), in.y.data(), in.w.data(), xyw.data()});
     });
     build_table(k1.data(), blocks.data(), nb, evOff.data(), E, k2dKeys, stride, table.data(), tileStart.data());
-    emu::launch(kCompPerSort * nb, kThreads, [&] {
-      k_density2d(blocks.data(), k1.data(), xyw.data(), i1.data(), table.data(), stride, dc, dc2, xyrp.data(),
-                  rho.data());
+    const int dv = argc > 3 ? atoi(argv[3]) : 0, nv = argc > 4 ? atoi(argv[4]) : 0;
+    dispatch(dv, [&](auto I, auto S) {
+      emu::launch(CompShape<I>::kPerSort * nb, kThreads, [&] {
+        k_density2d<I, S>(blocks.data(), k1.data(), xyw.data(), i1.data(), table.data(), stride, dc, dc2,
+                          xyrp.data(), rho.data());
+      });
     });
     int* parent = reinterpret_cast<int*>(k2.data());
-    emu::launch(kCompPerSort * nb, kThreads, [&] {
-      k_nh2d(blocks.data(), k1.data(), xyrp.data(), table.data(), stride, evOff.data(), dc, rhoc, odf, dm2,
-             delta.data(), nh.data(), isSeed.data(), parent);
+    dispatch(nv, [&](auto I, auto S) {
+      emu::launch(CompShape<I>::kPerSort * nb, kThreads, [&] {
+        k_nh2d<I, S>(blocks.data(), k1.data(), xyrp.data(), table.data(), stride, evOff.data(), dc, rhoc, odf, dm2,
+                     delta.data(), nh.data(), isSeed.data(), parent);
+      });
     });
     SeedRankHost seeds(N);
     launch_labels(blocks.data(), nb, parent, isSeed.data(), N, evOff.data(), ci.data(), seeds.view(), part.data());

@@ -29,7 +29,7 @@ def emulator():
     return EXE
 
 
-def run_emulated(exe, tmp_path, ev, offs, dc=3.0, rhoc=10.0, odf=2.0, wgs=2):
+def run_emulated(exe, tmp_path, ev, offs, dc=3.0, rhoc=10.0, odf=2.0, wgs=2, extra=()):
     N = int(offs[-1])
     src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(src, "wb") as f:
@@ -38,7 +38,8 @@ def run_emulated(exe, tmp_path, ev, offs, dc=3.0, rhoc=10.0, odf=2.0, wgs=2):
         f.write(np.asarray(offs, np.int64).tobytes())
         for k, dt in (("x", np.float32), ("y", np.float32), ("layer", np.int32), ("weight", np.float32)):
             f.write(np.ascontiguousarray(ev[k], dt).tobytes())
-    r = subprocess.run([str(exe), str(src), str(dst), str(wgs)], capture_output=True, text=True, timeout=600)
+    args = [str(exe), str(src), str(dst)] + ([str(wgs)] if exe == EXE else [str(a) for a in extra])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-4000:]
     d = np.fromfile(dst, np.int32)
     o = d[2:].reshape(5, N)
@@ -109,3 +110,14 @@ def test_emulated_global_batch(emulator, tmp_path, oracle_lib):
     err, _, got = run_emulated(EXE_GLOBAL, tmp_path, packed, offs)
     assert err == 0
     helpers.assert_bit_exact(got, oracle_lib.clue2d_batch(packed, offs), helpers.OUT2D, "emulated global batch")
+
+
+@pytest.mark.parametrize("variants", [(1, 1), (2, 3), (3, 2)])
+def test_emulated_global_variants(emulator, tmp_path, variants):
+    """Every (points per workgroup, LDS staging) variant of density and nearest higher gives the same bits."""
+    ins, want, extra = helpers.load_case("clue2d", "bin_edges")
+    dc, rhoc, odf = (float(v) for v in extra["params"])
+    err, _, got = run_emulated(EXE_GLOBAL, tmp_path, ins, np.array([0, ins["x"].size]), dc, rhoc, odf,
+                               extra=variants)
+    assert err == 0
+    helpers.assert_bit_exact(got, want, helpers.OUT2D, f"emulated variants {variants}")

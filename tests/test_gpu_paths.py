@@ -147,3 +147,29 @@ def test_set_path_rejects_bad_values(dev):
         dev.set_path(clue_hip.PATH_FUSED)  # disabled on the device
     with pytest.raises(clue_hip.ClueError):
         dev.set_path(clue_hip.PATH_FUSED, dim=3)
+
+
+def test_autotuned_plan(dev, oracle_lib):
+    """An autotuned plan: the tuning launch and the graph replays with the chosen variants are bit-exact."""
+    sizes = [200_000, 3_000, 60_000, 1_000]
+    packed, offs = synth.pack([synth.clue2d_event(n, 300 + i) for i, n in enumerate(sizes)])
+    N = int(offs[-1])
+    print(f"case clue2d autotuned plan n={N}", flush=True)
+    want = oracle_lib.clue2d_batch(packed, offs)
+    din = dev.upload_soa(packed, clue_hip.FIELDS_2D_IN)
+    dout = dev.alloc_soa(N, clue_hip.FIELDS_2D_OUT)
+    plan = dev.clue2d_plan(din, dout, offs, use_graph=True, autotune=True)
+    assert plan.tuning == {"density_points": 512, "density_stage": 1024, "nh_points": 512, "nh_stage": 1024}
+    for rep in range(3):
+        for a in dout.values():
+            a.upload(np.full(N, -7, a.dtype))
+        plan.launch()
+        dev.check()
+        helpers.assert_bit_exact(dev.download_soa(dout), want, helpers.OUT2D, f"autotuned launch {rep}")
+    t = plan.tuning
+    assert t["density_points"] in (256, 512, 1024) and t["density_stage"] in (1024, 2048), t
+    assert t["nh_points"] in (256, 512, 1024) and t["nh_stage"] in (1024, 2048), t
+    print(f"tuning {t}", flush=True)
+    plan.destroy()
+    for a in list(din.values()) + list(dout.values()):
+        a.free()
