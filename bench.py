@@ -103,6 +103,66 @@ def cpu_baseline(packed, offs, p, gpu_out):
             "seconds": round(t, 3), "labels_bit_exact_vs_gpu": bool(same)}
 
 
+def per_event_run3d(dev, din, dout, offs, p):
+    """CLUE3D: every event as its own launch sequence followed by a stream synchronisation."""
+    lib, ctx = dev.lib, dev.ctx
+    t0 = time.perf_counter()
+    for e in range(offs.size - 1):
+        a, b = int(offs[e]), int(offs[e + 1])
+        if b == a:
+            continue
+        rc = lib.clue3d_run_device(ctx, C.byref(p), b - a, *[ptr(din[k], a) for k in clue_hip.FIELDS_3D_IN],
+                                   *[ptr(dout[k], a) for k in clue_hip.FIELDS_3D_OUT], None)
+        if rc:
+            raise clue_hip.ClueError(rc, dev.error())
+        lib.clue_stream_sync(ctx, None)
+    t = time.perf_counter() - t0
+    dev.check()
+    return t
+
+
+def clue3d_leg(dev, n_events, steps, warmup):
+    """C4: 512 CLUE3D events of 50 layers x Poisson(200) layer clusters (S=2), batched graph vs per-event."""
+    packed, offs = synth.clue3d_batch(n_events)
+    N = int(offs[-1])
+    p = clue_hip.params3d(S=2)
+    din = dev.upload_soa(packed, clue_hip.FIELDS_3D_IN)
+    dout = dev.alloc_soa(N, clue_hip.FIELDS_3D_OUT)
+    plan = dev.clue3d_plan(din, dout, offs, p, use_graph=True)
+    for _ in range(max(1, warmup)):
+        plan.launch()
+    dev.check()
+    lib, ctx = dev.lib, dev.ctx
+    dev.sync()
+    lib.clue_mark(ctx, 2, None)
+    for _ in range(steps):
+        plan.launch()
+    lib.clue_mark(ctx, 3, None)
+    dev.sync()
+    dev.check()
+    ms = lib.clue_elapsed_ms(ctx, 2, 3) / steps
+    batch_out = dev.download_soa(dout)
+    pout = dev.alloc_soa(N, clue_hip.FIELDS_3D_OUT)
+    t = per_event_run3d(dev, din, pout, offs, p)
+    per_out = dev.download_soa(pout)
+    same = all(np.array_equal(per_out[k].view(np.int32), batch_out[k].view(np.int32)) for k in per_out)
+    plan.destroy()
+    for a in list(din.values()) + list(dout.values()) + list(pout.values()):
+        a.free()
+    value = N / (ms * 1e-3) / 1e6
+    algo = BYTES_PER_CLUSTER_3D * N
+    out = {"workload": f"clue3d_batched_{n_events}_events_50_layers_poisson200", "events": n_events, "clusters": N,
+           "densitySiblingLayers": 2, "ms_per_step": round(ms, 4), "value": round(value, 2),
+           "unit": "Mclusters/s", "per_event": {"value": round(N / t / 1e6, 2), "unit": "Mclusters/s",
+                                                "ms_total": round(t * 1e3, 3)},
+           "batched_over_per_event": round(value / (N / t / 1e6), 2), "labels_bit_exact_vs_per_event": bool(same),
+           "roofline": {"bound": "hbm", "achieved": round(algo / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "algorithmic_bytes_per_launch": algo}}
+    log(f"clue3d: {ms:.3f} ms/step ({value:.1f} Mclusters/s), per-event {t * 1e3:.1f} ms, bit-exact={same}")
+    return out
+
+
 def serving_run(dev, pool, order, arrivals, slots, max_events, p3):
     """Open-loop Poisson stream: submit each event at its arrival time, wait for all."""
     srv = clue_hip.Server(dev, p3=p3, max_points_2d=4_000_000, max_points_3d=1_000_000, max_events=max_events,
@@ -178,6 +238,7 @@ def main():
     ap.add_argument("--serve-rates", default="500,1500", help="serving leg: event rates (events/s), comma-separated")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="serving leg: length of the stream")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-3d", action="store_true", help="skip the CLUE3D (512-event) leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run 3 device-to-device copies of BYTES (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -251,6 +312,9 @@ def main():
                                "labels_bit_exact_vs_batched": bool(same)}
         result["batched_over_per_event"] = round(value / pe, 2)
         log(f"per-event: {t * 1e3:.2f} ms total ({pe:.1f} Mhits/s), bit-exact={same}")
+
+    if not args.no_3d:
+        result["clue3d"] = clue3d_leg(dev, 512, args.steps, args.warmup)
 
     if not args.no_serve:
         result["serving"] = [serving_leg(dev, float(r), args.serve_seconds) for r in args.serve_rates.split(",")]
