@@ -283,12 +283,13 @@ class Device:
                                                    *[dout[k].ptr for k in FIELDS_3D_OUT], stream))
 
     def clue3d_plan(self, din: dict, dout: dict, offs: np.ndarray, p: Clue3DParams | None = None,
-                    use_graph: bool = True) -> "Plan":
+                    use_graph: bool = True, autotune: bool = False) -> "Plan":
         p = p or params3d()
         offs = np.ascontiguousarray(offs, dtype=np.int64)
+        flags = (PLAN_GRAPH if use_graph else 0) | (PLAN_AUTOTUNE if autotune else 0)
         h = self.lib.clue3d_plan_create(self.ctx, C.byref(p), offs.size - 1, offs.ctypes.data,
                                         *[din[k].ptr for k in FIELDS_3D_IN], *[dout[k].ptr for k in FIELDS_3D_OUT],
-                                        1 if use_graph else 0)
+                                        flags)
         if not h:
             raise ClueError(-1, self.error())
         return Plan(self, h)

@@ -228,19 +228,27 @@ int plan_route_first(clue_plan* pl, hipStream_t s) {
 // fastest of each is kept. Re-running either kernel rewrites its outputs with the same values.
 int plan_autotune(clue_plan* pl, hipStream_t s) {
   clue_ctx* c = pl->ctx;
-  clue2d_enqueue(pl->b2, pl->io2, pl->p2, s);
+  const bool d3 = pl->dim == 3;
+  if (d3)
+    clue3d_enqueue(pl->b3, pl->io3, pl->p3, s);
+  else
+    clue2d_enqueue(pl->b2, pl->io2, pl->p2, s);
   HIP_TRY(c, hipGetLastError());
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIP_TRY(c, hipEventCreate(&e0));
   HIP_TRY(c, hipEventCreate(&e1));
+  const int nVariants = d3 ? kClue3DVariants : kClue2DVariants;
   int best[2] = {0, 0};
   for (int which = 0; which < 2; ++which) {
     float bestMs = 0.f;
-    for (int v = 0; v < kClue2DVariants; ++v) {
+    for (int v = 0; v < nVariants; ++v) {
       float ms = 0.f;
       for (int rep = 0; rep < 4; ++rep) {  // the first run of each variant warms it up
         HIP_TRY(c, hipEventRecord(e0, s));
-        clue2d_enqueue_step(pl->b2, pl->io2, pl->p2, which, v, s);
+        if (d3)
+          clue3d_enqueue_step(pl->b3, pl->io3, pl->p3, which, v, s);
+        else
+          clue2d_enqueue_step(pl->b2, pl->io2, pl->p2, which, v, s);
         HIP_TRY(c, hipEventRecord(e1, s));
         HIP_TRY(c, hipEventSynchronize(e1));
         float t = 0.f;
@@ -255,8 +263,13 @@ int plan_autotune(clue_plan* pl, hipStream_t s) {
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  pl->b2.densityVariant = best[0];
-  pl->b2.nhVariant = best[1];
+  if (d3) {
+    pl->b3.densityVariant = best[0];
+    pl->b3.nhVariant = best[1];
+  } else {
+    pl->b2.densityVariant = best[0];
+    pl->b2.nhVariant = best[1];
+  }
   pl->tune = false;
   return CLUE_OK;
 }
@@ -753,6 +766,7 @@ clue_plan* clue3d_plan_create(clue_ctx* c, const clue3d_params* p, int32_t E, co
   pl->io3 = Clue3DIO{z, eta, phi, roz, radius, energy, layer, isSi, rho, df, dl, nhl, nhi, isSeed, ti};
   pl->p3 = to3d(p);
   pl->useGraph = (use_graph & CLUE_PLAN_GRAPH) != 0;
+  pl->tune = (use_graph & CLUE_PLAN_AUTOTUNE) != 0;
   return pl;
 }
 
@@ -763,7 +777,7 @@ int32_t clue_plan_launch(clue_plan* pl, void* stream) {
   clue_ctx* c = pl->ctx;
   hipStream_t s = pick(c, stream);
   if (pl->dim == 2 && pl->route2d < 0) return plan_route_first(pl, s);
-  if (pl->dim == 2 && pl->route2d == CLUE_PATH_GLOBAL && pl->tune) return plan_autotune(pl, s);
+  if (pl->tune && (pl->dim == 3 || pl->route2d == CLUE_PATH_GLOBAL)) return plan_autotune(pl, s);
   if (!pl->useGraph) return launch_plan_body(pl, s);
   if (!pl->exec) {
     // Record the launch sequence once on a private stream, then replay it.
@@ -802,8 +816,13 @@ int32_t clue_plan_path(const clue_plan* pl) {
 int32_t clue_plan_tuning(const clue_plan* pl, int32_t* out4) {
   if (!pl || !out4) return CLUE_ERR_ARG;
   int di = 0, ds = 0, ni = 0, ns = 0;
-  clue2d_variant_shape(pl->b2.densityVariant, di, ds);
-  clue2d_variant_shape(pl->b2.nhVariant, ni, ns);
+  if (pl->dim == 3) {
+    clue3d_variant_shape(pl->b3.densityVariant, di, ds);
+    clue3d_variant_shape(pl->b3.nhVariant, ni, ns);
+  } else {
+    clue2d_variant_shape(pl->b2.densityVariant, di, ds);
+    clue2d_variant_shape(pl->b2.nhVariant, ni, ns);
+  }
   out4[0] = di * kThreads;
   out4[1] = ds;
   out4[2] = ni * kThreads;

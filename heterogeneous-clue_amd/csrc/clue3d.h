@@ -34,10 +34,8 @@ struct Clue3DBuffers {
   int* i1 = nullptr;
   uint32_t* k2 = nullptr;
   int* i2 = nullptr;
-  float* rozS = nullptr;
-  float* phiS = nullptr;
-  float* enS = nullptr;
-  float* rhoS = nullptr;
+  float4* cre = nullptr;  // sorted {r_over_absz, phi, energy, z}
+  float4* crp = nullptr;  // sorted {r_over_absz, phi, rho, z}
   int* hist = nullptr;
   int* part = nullptr;
   int* table = nullptr;
@@ -45,6 +43,8 @@ struct Clue3DBuffers {
   int64_t tableStride = 0;
   SeedRank seeds{};
   int* err = nullptr;
+  int densityVariant = 0;  // kernel shapes (points per thread, staged points) of the density and
+  int nhVariant = 0;       // nearest-higher steps; see clue3d_variant_shape
 };
 
 struct Clue3DIO {
@@ -68,6 +68,12 @@ struct Clue3DIO {
 constexpr int64_t clue3d_table_stride() { return ((int64_t)k3dKeys + 1 + 63) / 64 * 64; }
 
 void clue3d_enqueue(const Clue3DBuffers& b, const Clue3DIO& io, const Clue3DParams& p, hipStream_t s);
+// Re-runs one step of an enqueued batch with a given kernel variant: which = 0 density, 1 nearest
+// higher (and seed flags). Used by the plan autotuner, after a full clue3d_enqueue.
+void clue3d_enqueue_step(const Clue3DBuffers& b, const Clue3DIO& io, const Clue3DParams& p, int which, int variant,
+                         hipStream_t s);
+constexpr int kClue3DVariants = 4;
+void clue3d_variant_shape(int variant, int& items, int& stage);
 size_t clue3d_workspace_bytes(int64_t N, int nEvents, int nBlocks);
 void clue3d_carve(void* base, int64_t N, int E, int nBlocks, Clue3DBuffers& b);
 

@@ -166,4 +166,48 @@ __device__ __forceinline__ int block_exclusive_min(int v, int* s_tmp) {
   return excl < base ? excl : base;
 }
 
+// Block-wide [min lo, max hi).
+__device__ __forceinline__ void block_range(int lo, int hi, int* s_red, int& outLo, int& outHi) {
+  const int lane = lane_id(), wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const int ol = __shfl_xor(lo, off, kWave);
+    const int oh = __shfl_xor(hi, off, kWave);
+    lo = ol < lo ? ol : lo;
+    hi = oh > hi ? oh : hi;
+  }
+  if (lane == 0) {
+    s_red[wave] = lo;
+    s_red[kWaves + wave] = hi;
+  }
+  __syncthreads();
+  outLo = s_red[0];
+  outHi = s_red[kWaves];
+#pragma unroll
+  for (int w = 1; w < kWaves; ++w) {
+    outLo = s_red[w] < outLo ? s_red[w] : outLo;
+    outHi = s_red[kWaves + w] > outHi ? s_red[kWaves + w] : outHi;
+  }
+}
+
+// Copy [lo, hi) (1 <= hi - lo <= STAGE points) into LDS, all loads of a thread in flight together
+// (indices past the end re-read the last point and are not stored).
+template <int STAGE>
+__device__ __forceinline__ void stage_range(const float4* __restrict__ g, float4* st, int lo, int hi) {
+  static_assert(STAGE % kThreads == 0, "staging splits evenly over the block");
+  constexpr int kPer = STAGE / kThreads;
+  const int n = hi - lo;
+  float4 v[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int k = u * kThreads + (int)threadIdx.x;
+    v[u] = g[lo + (k < n ? k : n - 1)];
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int k = u * kThreads + (int)threadIdx.x;
+    if (k < n) st[k] = v[u];
+  }
+}
+
 }  // namespace clue

@@ -143,7 +143,7 @@ int32_t clue2d_run_batch_device(clue_ctx* ctx, const clue2d_params* p, int32_t n
 
 /* A reusable plan for a fixed batch (sizes and buffers): its own workspace, launched with
  * clue_plan_launch. use_graph is a set of flags: CLUE_PLAN_GRAPH records the launch sequence once into
- * a hipGraph and replays it; CLUE_PLAN_AUTOTUNE (CLUE2D) makes the first launch time every variant
+ * a hipGraph and replays it; CLUE_PLAN_AUTOTUNE makes the first launch time every variant
  * (points per workgroup, LDS staging capacity) of the density and nearest-higher kernels on the
  * plan's own data and keep the fastest of each for the later launches (clue_plan_tuning reports the
  * choice). All variants give the same bits. No reference counterpart. */
@@ -192,7 +192,7 @@ int32_t clue_plan_launch(clue_plan* plan, void* stream);
 /* CLUE2D plan: the path it runs (CLUE_PATH_FUSED or CLUE_PATH_GLOBAL), or -1 before the first launch
  * has decided it; CLUE3D plans: CLUE_PATH_GLOBAL. */
 int32_t clue_plan_path(const clue_plan* plan);
-/* CLUE2D plan: points per workgroup and LDS staging capacity (points) of the density and the
+/* Points per workgroup and LDS staging capacity (points) of the density and the
  * nearest-higher kernels it launches, as out[0..3] = {density points, density stage, nh points,
  * nh stage}. Before an autotuned plan's first launch these are the defaults. */
 int32_t clue_plan_tuning(const clue_plan* plan, int32_t* out4);

@@ -1,10 +1,12 @@
-"""The library's CLUE2D kernels run on the CPU through an emulation of the HIP execution model (one
+"""The library's CLUE2D and CLUE3D kernels run on the CPU through an emulation of the HIP execution model (one
 thread per lane, real barriers, wave ballots and shuffles through a per-wave mailbox,
 AddressSanitizer + UndefinedBehaviorSanitizer), with the library's own launch sequences and grids,
 bit-exact against the golden fixtures and the oracle:
 
 * the global-sort path (radix passes, bin table, density, nearest higher, seed ranks, labels), the
   path the device runs; device scratch starts poisoned, so reads of unwritten memory show up;
+* the CLUE3D path (radix passes, bin table, density and nearest higher over the sibling layers
+  with the staged run-list walk and the unstaged row scan, seed ranks, labels);
 * the fused segment path (layer bucket pass, segment lists, fused segment kernels, seed numbering),
   which the library keeps disabled on the device (clue_set_path refuses it) until it has also
   passed on the GPU."""
@@ -15,11 +17,13 @@ import numpy as np
 import pytest
 
 import helpers
+import oracle
 import synth
 
 EMU = Path(__file__).resolve().parent / "cpu_emu"
 EXE = EMU / "_build" / "fused2d_emu"
 EXE_GLOBAL = EMU / "_build" / "global2d_emu"
+EXE_3D = EMU / "_build" / "global3d_emu"
 
 
 @pytest.fixture(scope="module")
@@ -121,3 +125,74 @@ def test_emulated_global_variants(emulator, tmp_path, variants):
                                extra=variants)
     assert err == 0
     helpers.assert_bit_exact(got, want, helpers.OUT2D, f"emulated variants {variants}")
+
+
+# ------------------------------------------------------------------ CLUE3D
+
+FIELDS_3D_F = ("z", "eta", "phi", "r_over_absz", "radius", "energy")
+
+
+def run_emulated3d(tmp_path, ev, offs, params, variants=()):
+    N = int(offs[-1])
+    src, dst = tmp_path / "in3.bin", tmp_path / "out3.bin"
+    with open(src, "wb") as f:
+        f.write(np.int32(len(offs) - 1).tobytes())
+        f.write(bytes(params))
+        f.write(np.asarray(offs, np.int64).tobytes())
+        for k in FIELDS_3D_F:
+            f.write(np.ascontiguousarray(ev[k], np.float32).tobytes())
+        for k in ("layer", "isSilicon"):
+            f.write(np.ascontiguousarray(ev[k], np.int32).tobytes())
+    r = subprocess.run([str(EXE_3D), str(src), str(dst)] + [str(v) for v in variants], capture_output=True,
+                       text=True, timeout=900)
+    assert r.returncode == 0, r.stderr[-4000:]
+    d = np.fromfile(dst, np.int32)
+    o = d[1:].reshape(7, N)
+    out = {"rho": o[0].view(np.float32), "delta_f": o[1].view(np.float32), "delta_l": o[2], "nh_layer": o[3],
+           "nh_idx": o[4], "isSeed": o[5], "tracksterIndex": o[6]}
+    return int(d[0]), out
+
+
+@pytest.mark.parametrize("name", ["single", "phi_wrap", "eta_row34", "side_split", "scintillator", "ties3d",
+                                  "grid3d_ties", "c4_event1_S3"])
+def test_emulated_clue3d_golden(emulator, tmp_path, name):
+    ins, want, extra = helpers.load_case("clue3d", name)
+    err, got = run_emulated3d(tmp_path, ins, np.array([0, ins["layer"].size]),
+                              oracle.clue3d_params(S=int(extra["S"])))
+    assert err == 0
+    helpers.assert_bit_exact(got, want, helpers.OUT3D, f"emulated clue3d {name}")
+
+
+def test_emulated_clue3d_batch(emulator, tmp_path, oracle_lib):
+    """Three events of the C4 model at lower occupancy (one empty) in one batch, density over +/-2
+    sibling layers."""
+    evs = [synth.clue3d_event(e, n_layers=30, mean_per_layer=80.0) for e in range(2)]
+    evs.insert(1, {k: v[:0] for k, v in evs[0].items()})
+    packed, offs = synth.pack(evs)
+    p = oracle.clue3d_params(S=2)
+    err, got = run_emulated3d(tmp_path, packed, offs, p)
+    assert err == 0
+    helpers.assert_bit_exact(got, oracle_lib.clue3d_batch(packed, offs, p), helpers.OUT3D, "emulated clue3d batch")
+
+
+@pytest.mark.parametrize("variants", [(1, 2), (2, 3), (3, 1)])
+def test_emulated_clue3d_variants(emulator, tmp_path, oracle_lib, variants):
+    """Every (points per workgroup, LDS staging) variant gives the same bits; the 1024-point stage
+    leaves most blocks of this event unstaged, so the row scan from global memory runs too."""
+    ev = synth.clue3d_event(5, n_layers=25)
+    p = oracle.clue3d_params(S=2)
+    err, got = run_emulated3d(tmp_path, ev, np.array([0, ev["layer"].size]), p, variants)
+    assert err == 0
+    helpers.assert_bit_exact(got, oracle_lib.clue3d(ev, p), helpers.OUT3D, f"emulated clue3d variants {variants}")
+
+
+@pytest.mark.parametrize("kw", [{"S": 3, "densityOnSameLayer": 1, "nearestHigherOnSameLayer": 1},
+                                {"S": 0}, {"S": 40}, {"S": 1, "nearestHigherOnSameLayer": 1}])
+def test_emulated_clue3d_params(emulator, tmp_path, oracle_lib, kw):
+    """Same-layer density / nearest higher, no sibling layers, and a reach beyond the run list's layer
+    field (every block scans from global memory)."""
+    ev = synth.clue3d_event(7, n_layers=60, mean_per_layer=60.0)
+    p = oracle.clue3d_params(**kw)
+    err, got = run_emulated3d(tmp_path, ev, np.array([0, ev["layer"].size]), p)
+    assert err == 0
+    helpers.assert_bit_exact(got, oracle_lib.clue3d(ev, p), helpers.OUT3D, f"emulated clue3d {kw}")

@@ -72,6 +72,44 @@ def test_plan_graph_c4_properties(dev, oracle_lib):
     plan.destroy()
 
 
+def test_autotuned_plan(dev, oracle_lib):
+    """An autotuned CLUE3D plan: the tuning launch and the graph replays with the chosen variants are
+    bit-exact."""
+    packed, offs = synth.clue3d_batch(24)
+    N = int(offs[-1])
+    print(f"case clue3d autotuned plan n={N}", flush=True)
+    p = oracle.clue3d_params(S=2)
+    want = oracle_lib.clue3d_batch(packed, offs, p)
+    din = dev.upload_soa(packed, clue_hip.FIELDS_3D_IN)
+    dout = dev.alloc_soa(N, clue_hip.FIELDS_3D_OUT)
+    plan = dev.clue3d_plan(din, dout, offs, clue_hip.params3d(S=2), use_graph=True, autotune=True)
+    assert plan.tuning == {"density_points": 512, "density_stage": 2048, "nh_points": 512, "nh_stage": 2048}
+    for rep in range(3):
+        for a in dout.values():
+            a.upload(np.full(N, -7, a.dtype))
+        plan.launch()
+        dev.check()
+        helpers.assert_bit_exact(dev.download_soa(dout), want, helpers.OUT3D, f"autotuned launch {rep}")
+    t = plan.tuning
+    assert t["density_points"] in (256, 512, 1024) and t["density_stage"] in (1024, 2048), t
+    assert t["nh_points"] in (256, 512, 1024) and t["nh_stage"] in (1024, 2048), t
+    print(f"tuning {t}", flush=True)
+    plan.destroy()
+    for a in list(din.values()) + list(dout.values()):
+        a.free()
+
+
+@pytest.mark.parametrize("kw", [{"S": 3, "densityOnSameLayer": 1, "nearestHigherOnSameLayer": 1},
+                                {"S": 0}, {"S": 40}, {"S": 1, "nearestHigherOnSameLayer": 1}])
+def test_parameters(dev, oracle_lib, kw):
+    """Same-layer density / nearest higher, no sibling layers, and a sibling reach beyond the staged
+    run lists (every block scans its windows from global memory)."""
+    ev = synth.clue3d_event(7, n_layers=60, mean_per_layer=60.0)
+    print(f"case clue3d parameters {kw} n={ev['layer'].size}", flush=True)
+    got = dev.clue3d_host(ev, clue_hip.params3d(**kw))
+    helpers.assert_bit_exact(got, oracle_lib.clue3d(ev, oracle.clue3d_params(**kw)), helpers.OUT3D, f"{kw}")
+
+
 def test_invalid_layer_is_reported(dev):
     ev = synth.clue3d_event(2)
     ev["layer"][5] = 94
