@@ -137,6 +137,25 @@ __device__ __forceinline__ void win_reach(const int* __restrict__ T, const Win3D
   hi = T[w.maxL * k3dBinsPerLayer + w.emax * k3dPhi + (wrap ? k3dPhi : w.b0 + 1)];
 }
 
+// reco::reduceRange (Math/deltaPhi.h:17-24) returns its argument unchanged when |x| <= pi (the
+// float nearest M_PI). Candidates of one window differ in phi by less than that unless the window
+// wraps or a phi lies outside [-pi, pi], so the candidate steps test both differences of a pair and
+// take the reduction only when one of them needs it.
+constexpr float kPiF = (float)3.14159265358979323846;
+__device__ __forceinline__ bool phi_wide(float x) { return !(fabsf(x) <= kPiF); }
+__device__ __forceinline__ float reduce_wide(float x) {  // reduce_range for |x| > pi or NaN
+  const float o2pi = (float)(1. / (2. * 3.14159265358979323846));
+  const float n = roundf(x * o2pi);
+  return x - n * (float)(2. * 3.14159265358979323846);
+}
+__device__ __forceinline__ void reduce_pair(float& dp0, float& dp1) {
+  const bool w0 = phi_wide(dp0), w1 = phi_wide(dp1);
+  if (__builtin_expect(w0 | w1, 0)) {
+    dp0 = w0 ? reduce_wide(dp0) : dp0;
+    dp1 = w1 ? reduce_wide(dp1) : dp1;
+  }
+}
+
 // ---------------------------------------------------------------- kernel shapes
 
 // The density and nearest-higher kernels give each block ITEMS * kThreads consecutive sorted points
@@ -185,8 +204,17 @@ __device__ __forceinline__ int list_layer(const int* __restrict__ T, const Win3D
     const int* Tr = TL + ie * k3dPhi;
     s0[r] = Tr[w.a0];
     t0[r] = Tr[w.b0 + 1];
-    s1[r] = Tr[w.a1];      // a1 = 0, b1 = -1 without a second run: [Tr[0], Tr[0]) is empty
-    t1[r] = Tr[w.b1 + 1];
+    s1[r] = 0;
+    t1[r] = 0;
+  }
+  if (win_wraps(w)) {  // the second run of each row (phi bins 0..b1) only exists for wrapped windows
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const int ie = w.emin + r <= w.emax ? w.emin + r : w.emax;
+      const int* Tr = TL + ie * k3dPhi;
+      s1[r] = Tr[w.a1];
+      t1[r] = Tr[w.b1 + 1];
+    }
   }
   const int off = cl - w.minL;
 #pragma unroll
@@ -276,9 +304,8 @@ struct DensityStep3D {
   const float4* src;
   float rho;
   __device__ __forceinline__ void run(int layerOff) { kdfRun = (minL + layerOff == L) ? kdfSame : kdf; }
-  __device__ __forceinline__ float term(const float4& c, int q) const {
+  __device__ __forceinline__ float term(const float4& c, float dp, int q) const {
     const float r1 = c.x * zi;
-    const float dp = reduce_range(phii - c.y);
     const float d = (r0 - r1) * (r0 - r1) + r1 * r1 * dp * dp;
     const float e = ((q == p) ? 1.f : kdfRun) * c.z;
     return d < dsq ? e : 0.f;
@@ -286,12 +313,17 @@ struct DensityStep3D {
   __device__ __forceinline__ void operator()(int q0, int q1, bool two) {
     const float4 c0 = src[q0];
     const float4 c1 = src[q1];
-    const float e0 = term(c0, q0);
-    const float e1 = term(c1, q1);
+    float dp0 = phii - c0.y, dp1 = phii - c1.y;
+    reduce_pair(dp0, dp1);
+    const float e0 = term(c0, dp0, q0);
+    const float e1 = term(c1, dp1, q1);
     rho += e0;
     rho += two ? e1 : 0.f;
   }
-  __device__ __forceinline__ void one(int q) { rho += term(src[q], q); }
+  __device__ __forceinline__ void one(int q) {
+    const float4 c = src[q];
+    rho += term(c, reduce_range(phii - c.y), q);
+  }
 };
 
 // rho of every point (KernelCalculateDensitySoA). cre: sorted {r_over_absz, phi, energy, z};
@@ -368,9 +400,8 @@ struct NhStep3D {
   float best;
   int bq, bo, cur;
   __device__ __forceinline__ void run(int layerOff) { cur = layerOff; }
-  __device__ __forceinline__ void take(const float4& c, int q, bool valid) {
+  __device__ __forceinline__ void take(const float4& c, float dp, int q, bool valid) {
     const float r1 = c.x * zi;
-    const float dp = reduce_range(phii - c.y);
     const float d = (r0 - r1) * (r0 - r1) + r1 * r1 * dp * dp;
     // bitwise forms: lane masks combined without branches
     const bool upd = valid & (c.z > rhoi) & (d <= best);
@@ -381,10 +412,15 @@ struct NhStep3D {
   __device__ __forceinline__ void operator()(int q0, int q1, bool two) {
     const float4 c0 = src[q0];
     const float4 c1 = src[q1];
-    take(c0, q0, true);
-    take(c1, q1, two);
+    float dp0 = phii - c0.y, dp1 = phii - c1.y;
+    reduce_pair(dp0, dp1);
+    take(c0, dp0, q0, true);
+    take(c1, dp1, q1, two);
   }
-  __device__ __forceinline__ void one(int q) { take(src[q], q, true); }
+  __device__ __forceinline__ void one(int q) {
+    const float4 c = src[q];
+    take(c, reduce_range(phii - c.y), q, true);
+  }
 };
 
 // delta / nearestHigher (KernelComputeDistanceToHigherSoA) plus the seed / outlier decision of
