@@ -181,20 +181,38 @@ __global__ __launch_bounds__(kThreads) void k_labels(const BlockDesc* __restrict
                                                      int32_t* __restrict__ label) {
   const BlockDesc bd = blocks[xcd_block((int)blockIdx.x, (int)gridDim.x)];
   const int rank0 = rank_of(sr, evOff[bd.event]);
-#pragma unroll 1
+  // The thread's kSortItems points walk to their roots together: each round issues the next parent
+  // read of every unfinished walk before using any, so the dependent loads of different walks
+  // overlap. Every value read from parent[] (old or already shortened by another thread) is an
+  // ancestor of the point read, and roots never change, so each walk ends.
+  int cur[kSortItems];
+  bool walking[kSortItems];
+#pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     const int i = bd.begin + r * kThreads + (int)threadIdx.x;
-    if (i >= bd.end) break;
-    // Walk to the root. Every value read from parent[] (old or already shortened by another
-    // thread) is an ancestor of the point read, and roots never change, so the walk ends.
-    int cur = parent[i];
-    for (;;) {
-      const int nxt = parent[cur];
-      if (nxt == cur) break;
-      cur = nxt;
+    walking[r] = i < bd.end;
+    cur[r] = walking[r] ? parent[i] : 0;
+  }
+  bool any = true;
+  while (any) {
+    int nxt[kSortItems];
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) nxt[r] = walking[r] ? parent[cur[r]] : cur[r];
+    any = false;
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+      walking[r] = nxt[r] != cur[r];
+      cur[r] = nxt[r];
+      any |= walking[r];
     }
-    parent[i] = cur;
-    label[i] = seed_of(sr, cur) ? rank_of(sr, cur) - rank0 : -1;
+  }
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int i = bd.begin + r * kThreads + (int)threadIdx.x;
+    if (i < bd.end) {
+      parent[i] = cur[r];
+      label[i] = seed_of(sr, cur[r]) ? rank_of(sr, cur[r]) - rank0 : -1;
+    }
   }
 }
 

@@ -42,6 +42,23 @@ def test_one_million_hit_event_vs_oracle(dev, oracle_lib):
     helpers.check_invariants_2d(ev, got, 3.0, 10.0, 2.0)
 
 
+def test_dense_column_of_70000_hits(dev, oracle_lib):
+    """70,000 hits in one 5 cm column of one layer (about 1400 candidates per hit, bin-table runs far
+    beyond 16 bits), plus a C1-model event on other layers."""
+    rng = np.random.Generator(np.random.PCG64(77))
+    n = 70_000
+    base = synth.clue2d_event(2_000, 77)
+    base["layer"] = (base["layer"] % 50 + 50).astype(np.int32)
+    ev = {"x": np.concatenate([rng.uniform(0.5, 4.5, n).astype(np.float32), base["x"]]),
+          "y": np.concatenate([rng.uniform(-249.0, 249.0, n).astype(np.float32), base["y"]]),
+          "layer": np.concatenate([np.full(n, 7, np.int32), base["layer"]]),
+          "weight": np.concatenate([rng.exponential(1.0, n).astype(np.float32), base["weight"]])}
+    print(f"case clue2d dense column n={ev['x'].size}", flush=True)
+    got = dev.clue2d_host(ev)
+    want = oracle_lib.clue2d(ev["x"], ev["y"], ev["layer"], ev["weight"])
+    helpers.assert_bit_exact(got, want, helpers.OUT2D, "dense column")
+
+
 def mixed_batch():
     sizes = [1000, 0, 1, 23_456, 150_000, 3_333, 70_000, 2_048, 2_049, 120]
     evs = [synth.clue2d_event(n, 500 + i, shuffle=(i % 3 == 0)) for i, n in enumerate(sizes)]

@@ -110,6 +110,27 @@ def test_parameters(dev, oracle_lib, kw):
     helpers.assert_bit_exact(got, oracle_lib.clue3d(ev, oracle.clue3d_params(**kw)), helpers.OUT3D, f"{kw}")
 
 
+def test_dense_row_of_70000_clusters(dev, oracle_lib):
+    """70,000 layer clusters in one (layer, eta) row (about 2800 candidates per cluster, blocks too wide
+    to stage), plus a C4-model event."""
+    rng = np.random.Generator(np.random.PCG64(78))
+    n = 70_000
+    eta = rng.uniform(1.56, 1.59, n).astype(np.float32)
+    phi = rng.uniform(-np.pi, np.pi, n).astype(np.float32)
+    layer = np.full(n, 12, np.int32)
+    z = (320.0 + 1.2 * layer.astype(np.float64)).astype(np.float32)
+    roz = (1.0 / np.sinh(eta.astype(np.float64))).astype(np.float32)
+    rr = z.astype(np.float64) * roz.astype(np.float64)
+    dense = {"x": (rr * np.cos(phi)).astype(np.float32), "y": (rr * np.sin(phi)).astype(np.float32), "z": z,
+             "eta": eta, "phi": phi, "r_over_absz": roz, "radius": np.full(n, 1.8, np.float32), "layer": layer,
+             "energy": rng.exponential(2.0, n).astype(np.float32), "isSilicon": np.ones(n, np.int32)}
+    other = synth.clue3d_event(9)
+    ev = {k: np.concatenate([dense[k], other[k]]) for k in other}
+    print(f"case clue3d dense row n={ev['layer'].size}", flush=True)
+    got = dev.clue3d_host(ev, clue_hip.params3d(S=2))
+    helpers.assert_bit_exact(got, oracle_lib.clue3d(ev, oracle.clue3d_params(S=2)), helpers.OUT3D, "dense row")
+
+
 def test_invalid_layer_is_reported(dev):
     ev = synth.clue3d_event(2)
     ev["layer"][5] = 94
