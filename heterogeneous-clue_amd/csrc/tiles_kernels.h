@@ -113,10 +113,22 @@ __global__ __launch_bounds__(kThreads) void k_table(const uint32_t* __restrict__
   }
   for (int j = threadIdx.x; j < kTableTile; j += kThreads) s_first[tpad(j)] = 0x7fffffff;
   __syncthreads();
-  // first position of every non-empty bin of the tile (one writer per bin: the run start)
-  for (int p = lo + (int)threadIdx.x; p < hi; p += kThreads) {
-    const uint32_t k = key[p];
-    if (p == lo || key[p - 1] != k) s_first[tpad((int)(k - (uint32_t)b0))] = p;
+  // first position of every non-empty bin of the tile (one writer per bin: the run start); four
+  // rounds of key loads in flight together
+  constexpr int kU = 4;
+  for (int p0 = lo; p0 < hi; p0 += kU * kThreads) {
+    uint32_t k[kU], kp[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int p = p0 + u * kThreads + (int)threadIdx.x;
+      k[u] = p < hi ? key[p] : 0u;
+      kp[u] = p < hi && p > lo ? key[p - 1] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int p = p0 + u * kThreads + (int)threadIdx.x;
+      if (p < hi && (p == lo || kp[u] != k[u])) s_first[tpad((int)(k[u] - (uint32_t)b0))] = p;
+    }
   }
   __syncthreads();
   // table[b] = min(first of bins >= b, hi): a suffix minimum. Thread t owns bins
