@@ -34,8 +34,8 @@ struct Clue2DBuffers {
   int* i1 = nullptr;
   uint32_t* k2 = nullptr;
   int* i2 = nullptr;
-  float4* xyw = nullptr;   // tile order: {x, y, weight, 0}
-  float4* xyrp = nullptr;  // tile order: {x, y, rho, point index as int bits}
+  float4* xyw = nullptr;   // tile order: {x, y, weight, point index as int bits}
+  float* rhoS = nullptr;   // tile order: rho
   int* hist = nullptr;
   int* part = nullptr;
   int* table = nullptr;

@@ -103,15 +103,14 @@ void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DPara
   scan_exclusive(b.hist, b.hist, nHist, b.part, s);
   k_radix_scatter<KeyArray, DigitL, EmitLast2D><<<b.nBlocks, kThreads, 0, s>>>(
       b.blocks, KeyArray{b.k2}, b.hist,
-      EmitLast2D{b.i2, b.k1, b.i1, io.x, io.y, io.weight, b.xyw});
+      EmitLast2D{b.i2, b.k1, io.x, io.y, io.weight, b.xyw});
   const uint32_t* keyS = b.k1;
-  const int* perm = b.i1;
   build_table(keyS, b.blocks, b.nBlocks, b.evOff, b.E, k2dKeys, b.tableStride, b.table, b.tileStart, s);
-  launch_density2d(b.densityVariant, b.nBlocks, s, b.blocks, keyS, b.xyw, perm, b.table, b.tableStride, prm.dc,
-                   prm.dc2, b.xyrp, io.rho);
+  launch_density2d(b.densityVariant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.table, b.tableStride, prm.dc,
+                   prm.dc2, b.rhoS, io.rho);
   int* parent = reinterpret_cast<int*>(b.k2);
-  launch_nh2d(b.nhVariant, b.nBlocks, s, b.blocks, keyS, b.xyrp, b.table, b.tableStride, b.evOff, prm.dc, prm.rhoc,
-              prm.outlierDeltaFactor, prm.dm2, io.delta, io.nearestHigher, io.isSeed, parent);
+  launch_nh2d(b.nhVariant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.rhoS, b.table, b.tableStride, b.evOff, prm.dc,
+              prm.rhoc, prm.outlierDeltaFactor, prm.dm2, io.delta, io.nearestHigher, io.isSeed, parent);
   launch_labels(b.blocks, b.nBlocks, parent, io.isSeed, b.N, b.evOff, io.clusterIndex, b.seeds, b.part, s);
 }
 
@@ -119,14 +118,13 @@ void clue2d_enqueue_step(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2
                          hipStream_t s) {
   if (b.N <= 0 || b.nBlocks <= 0) return;
   const uint32_t* keyS = b.k1;
-  const int* perm = b.i1;
   int* parent = reinterpret_cast<int*>(b.k2);
   if (which == 0)
-    launch_density2d(variant, b.nBlocks, s, b.blocks, keyS, b.xyw, perm, b.table, b.tableStride, prm.dc, prm.dc2,
-                     b.xyrp, io.rho);
+    launch_density2d(variant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.table, b.tableStride, prm.dc, prm.dc2,
+                     b.rhoS, io.rho);
   else
-    launch_nh2d(variant, b.nBlocks, s, b.blocks, keyS, b.xyrp, b.table, b.tableStride, b.evOff, prm.dc, prm.rhoc,
-                prm.outlierDeltaFactor, prm.dm2, io.delta, io.nearestHigher, io.isSeed, parent);
+    launch_nh2d(variant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.rhoS, b.table, b.tableStride, b.evOff, prm.dc,
+                prm.rhoc, prm.outlierDeltaFactor, prm.dm2, io.delta, io.nearestHigher, io.isSeed, parent);
 }
 
 void clue2d_variant_shape(int variant, int& items, int& stage) {
@@ -147,7 +145,7 @@ void clue2d_carve(void* base, int64_t N, int E, int nBlocks, Clue2DBuffers& b) {
   b.k2 = cv.take<uint32_t>(n);
   b.i2 = cv.take<int>(n);
   b.xyw = cv.take<float4>(n);
-  b.xyrp = cv.take<float4>(n);
+  b.rhoS = cv.take<float>(n);
   b.hist = cv.take<int>((size_t)kRadix * (nBlocks > 0 ? nBlocks : 1));
   const int64_t maxScan = (int64_t)kRadix * nBlocks > N ? (int64_t)kRadix * nBlocks : N;
   b.part = cv.take<int>((size_t)scan_partials(maxScan > 0 ? maxScan : 1));
@@ -164,7 +162,8 @@ size_t clue2d_workspace_bytes(int64_t N, int nEvents, int nBlocks) {
   size_t bytes = 0;
   auto add = [&](size_t sz) { bytes += (sz + 255) & ~(size_t)255; };
   for (int k = 0; k < 4; ++k) add(n * 4);                   // k1 i1 k2 i2
-  for (int k = 0; k < 2; ++k) add(n * 16);                  // xyw xyrp
+  add(n * 16);                                              // xyw
+  add(n * 4);                                               // rhoS
   add((size_t)kRadix * (size_t)(nBlocks > 0 ? nBlocks : 1) * 4);  // hist
   const int64_t maxScan = (int64_t)kRadix * nBlocks > N ? (int64_t)kRadix * nBlocks : N;
   add((size_t)scan_partials(maxScan > 0 ? maxScan : 1) * 4);  // partials

@@ -39,11 +39,11 @@ struct Keys2D {
   }
 };
 
-// Last pass: also gathers x, y, weight into tile order.
+// Last pass: also gathers x, y, weight and the point index into tile order as {x, y, w, index bits}
+// (the permutation itself is not stored).
 struct EmitLast2D {
   const int* iIn;
   uint32_t* kOut;
-  int* iOut;
   const float* x;
   const float* y;
   const float* w;
@@ -58,8 +58,7 @@ struct EmitLast2D {
   }
   __device__ __forceinline__ void store(int pos, uint32_t k, const Payload& v) const {
     kOut[pos] = k;
-    iOut[pos] = v.idx;
-    xyw[pos] = make_float4(v.x, v.y, v.w, 0.f);
+    xyw[pos] = make_float4(v.x, v.y, v.w, __int_as_float(v.idx));
   }
 };
 
@@ -126,6 +125,46 @@ __device__ __forceinline__ float4 cand(const float4* __restrict__ g, const float
     return st[q - lo];
   else
     return g[q];
+}
+
+// Nearest-higher candidates {x, y, rho, index bits}: x, y and the index from the tile-order
+// {x, y, w, index} records, rho from the tile-order rho array the density step writes.
+struct NhSrc {
+  const float4* xyw;
+  const float* rho;
+  __device__ __forceinline__ float4 at(int q) const {
+    const float4 a = xyw[q];
+    return make_float4(a.x, a.y, rho[q], a.w);
+  }
+};
+template <bool STAGED>
+__device__ __forceinline__ float4 cand(const NhSrc& g, const float4* st, int q, int lo) {
+  if constexpr (STAGED)
+    return st[q - lo];
+  else
+    return g.at(q);
+}
+
+// stage_range for the nearest-higher candidates.
+template <int STAGE>
+__device__ __forceinline__ void stage_range_nh(const NhSrc& g, float4* st, int lo, int hi) {
+  static_assert(STAGE % kThreads == 0, "staging splits evenly over the block");
+  constexpr int kPer = STAGE / kThreads;
+  const int n = hi - lo;
+  float4 v[kPer];
+  float r[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int k = u * kThreads + (int)threadIdx.x;
+    const int q = lo + (k < n ? k : n - 1);
+    v[u] = g.xyw[q];
+    r[u] = g.rho[q];
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int k = u * kThreads + (int)threadIdx.x;
+    if (k < n) st[k] = make_float4(v[u].x, v[u].y, r[u], v[u].w);
+  }
 }
 
 // Run k >= kMaxCols of a box: sorted positions [s, t) of column xlo + k, read from the table.
@@ -219,9 +258,9 @@ __device__ __forceinline__ float density_run(int p, float xi, float yi, int s, i
 template <int ITEMS, int STAGE>
 __global__ __launch_bounds__(kThreads) void k_density2d(const BlockDesc* __restrict__ blocks,
                                                         const uint32_t* __restrict__ keyS,
-                                                        const float4* __restrict__ xyw, const int* __restrict__ perm,
+                                                        const float4* __restrict__ xyw,
                                                         const int* __restrict__ table, int64_t stride, float dc,
-                                                        float dc2, float4* __restrict__ xyrp,
+                                                        float dc2, float* __restrict__ rhoS,
                                                         float* __restrict__ rhoOut) {
   using Shape = CompShape<ITEMS>;
   __shared__ float4 s_st[STAGE];
@@ -271,8 +310,8 @@ __global__ __launch_bounds__(kThreads) void k_density2d(const BlockDesc* __restr
         rho = staged ? density_run<true>(p, xi, yi, s, t, xyw, s_st, rlo, dc2, rho)
                      : density_run<false>(p, xi, yi, s, t, xyw, s_st, rlo, dc2, rho);
       }
-      const int i = perm[p];
-      xyrp[p] = make_float4(xi, yi, rho, __int_as_float(i));
+      const int i = __float_as_int(me[r].w);
+      rhoS[p] = rho;
       rhoOut[i] = rho;
     }
   }
@@ -287,11 +326,11 @@ __global__ __launch_bounds__(kThreads) void k_density2d(const BlockDesc* __restr
 // with it and the final minimum is the reference's answer. In the tie case the scan is redone
 // exactly as the reference does it.
 template <bool STAGED>
-__device__ __forceinline__ void nh_run(int i, float xi, float yi, float rhoi, int s, int t,
-                                       const float4* __restrict__ xyrp, const float4* st, int lo, float dm2,
+__device__ __forceinline__ void nh_run(int i, float xi, float yi, float rhoi, int s, int t, const NhSrc& src,
+                                       const float4* st, int lo, float dm2,
                                        float& best2, int& best, float& prev2, int& prev) {
   for (int q = s; q < t; ++q) {
-    const float4 c = cand<STAGED>(xyrp, st, q, lo);
+    const float4 c = cand<STAGED>(src, st, q, lo);
     const float dx = xi - c.x;
     const float dy = yi - c.y;
     const float d2 = dx * dx + dy * dy;
@@ -312,7 +351,7 @@ template <bool STAGED>
 struct NhStep {
   int i;
   float xi, yi, rhoi, dm2;
-  const float4* g;
+  NhSrc g;
   const float4* st;
   int lo;
   float best2, prev2;
@@ -340,10 +379,10 @@ struct NhStep {
 
 template <bool STAGED>
 __device__ __forceinline__ void nh_run_exact(int i, float xi, float yi, float rhoi, int s, int t,
-                                             const float4* __restrict__ xyrp, const float4* st, int lo, float dm2,
+                                             const NhSrc& src, const float4* st, int lo, float dm2,
                                              float& delta, int& nh) {
   for (int q = s; q < t; ++q) {
-    const float4 c = cand<STAGED>(xyrp, st, q, lo);
+    const float4 c = cand<STAGED>(src, st, q, lo);
     const float dx = xi - c.x;
     const float dy = yi - c.y;
     const float d2 = dx * dx + dy * dy;
@@ -360,16 +399,16 @@ __device__ __forceinline__ void nh_run_exact(int i, float xi, float yi, float rh
 
 template <bool STAGED>
 __device__ __forceinline__ int nearest_higher_of(int i, float xi, float yi, float rhoi, const int* __restrict__ T,
-                                                 const Runs& runs, const float4* __restrict__ xyrp,
+                                                 const Runs& runs, const NhSrc& src,
                                                  const float4* st, int lo, float dm2, float& delta) {
-  NhStep<STAGED> f{i, xi, yi, rhoi, dm2, xyrp, st, lo, INFINITY, INFINITY, -1, -1};
+  NhStep<STAGED> f{i, xi, yi, rhoi, dm2, src, st, lo, INFINITY, INFINITY, -1, -1};
   walk_runs(runs, f);
   float best2 = f.best2, prev2 = f.prev2;
   int best = f.best, prev = f.prev;
   for (int k = kMaxCols; k < runs.n; ++k) {
     int s, t;
     run_bounds(T, runs, k, s, t);
-    nh_run<STAGED>(i, xi, yi, rhoi, s, t, xyrp, st, lo, dm2, best2, best, prev2, prev);
+    nh_run<STAGED>(i, xi, yi, rhoi, s, t, src, st, lo, dm2, best2, best, prev2, prev);
   }
   delta = FLT_MAX;
   if (best < 0) return -1;
@@ -380,7 +419,7 @@ __device__ __forceinline__ int nearest_higher_of(int i, float xi, float yi, floa
   for (int k = 0; k < runs.n; ++k) {
     int s, t;
     run_bounds(T, runs, k, s, t);
-    nh_run_exact<STAGED>(i, xi, yi, rhoi, s, t, xyrp, st, lo, dm2, delta, nh);
+    nh_run_exact<STAGED>(i, xi, yi, rhoi, s, t, src, st, lo, dm2, delta, nh);
   }
   return nh;
 }
@@ -390,7 +429,8 @@ __device__ __forceinline__ int nearest_higher_of(int i, float xi, float yi, floa
 template <int ITEMS, int STAGE>
 __global__ __launch_bounds__(kThreads) void k_nh2d(const BlockDesc* __restrict__ blocks,
                                                    const uint32_t* __restrict__ keyS,
-                                                   const float4* __restrict__ xyrp, const int* __restrict__ table,
+                                                   const float4* __restrict__ xyw, const float* __restrict__ rhoS,
+                                                   const int* __restrict__ table,
                                                    int64_t stride, const int* __restrict__ evOff, float dc, float rhoc,
                                                    float odf, float dm2, float* __restrict__ deltaOut,
                                                    int32_t* __restrict__ nhOut, int32_t* __restrict__ isSeedOut,
@@ -405,6 +445,7 @@ __global__ __launch_bounds__(kThreads) void k_nh2d(const BlockDesc* __restrict__
   const int* T = table + (int64_t)bd.event * stride;
   const int evBase = evOff[bd.event];
   const float dm = odf * dc;
+  const NhSrc src{xyw, rhoS};
   Runs runs[ITEMS];
   float4 me[ITEMS];
   int lo = 0x7fffffff, hi = -1;
@@ -412,7 +453,7 @@ __global__ __launch_bounds__(kThreads) void k_nh2d(const BlockDesc* __restrict__
   for (int r = 0; r < ITEMS; ++r) {
     const int p = b0 + r * kThreads + (int)threadIdx.x;
     if (p < b1) {
-      me[r] = xyrp[p];
+      me[r] = src.at(p);
       make_runs(T, keyS[p], me[r].x, me[r].y, dm, runs[r]);
       const int l = runs_lo(runs[r]), h = runs_hi(T, runs[r]);
       lo = l < lo ? l : lo;
@@ -422,7 +463,7 @@ __global__ __launch_bounds__(kThreads) void k_nh2d(const BlockDesc* __restrict__
   int rlo, rhi;
   block_range(lo, hi, s_red, rlo, rhi);
   const bool staged = rhi - rlo <= STAGE;
-  if (staged) stage_range<STAGE>(xyrp, s_st, rlo, rhi);
+  if (staged) stage_range_nh<STAGE>(src, s_st, rlo, rhi);
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
@@ -432,8 +473,8 @@ __global__ __launch_bounds__(kThreads) void k_nh2d(const BlockDesc* __restrict__
       const float rhoi = me[r].z;
       float delta;
       const int nhg = staged
-                          ? nearest_higher_of<true>(i, me[r].x, me[r].y, rhoi, T, runs[r], xyrp, s_st, rlo, dm2, delta)
-                          : nearest_higher_of<false>(i, me[r].x, me[r].y, rhoi, T, runs[r], xyrp, s_st, rlo, dm2,
+                          ? nearest_higher_of<true>(i, me[r].x, me[r].y, rhoi, T, runs[r], src, s_st, rlo, dm2, delta)
+                          : nearest_higher_of<false>(i, me[r].x, me[r].y, rhoi, T, runs[r], src, s_st, rlo, dm2,
                                                      delta);
       deltaOut[i] = delta;
       nhOut[i] = nhg >= 0 ? nhg - evBase : -1;

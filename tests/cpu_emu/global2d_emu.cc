@@ -45,7 +45,8 @@ int main(int argc, char** argv) {
   const int64_t maxScan = std::max<int64_t>((int64_t)kRadix * nb, N);
   std::vector<int> part(scan_partials(std::max<int64_t>(maxScan, 1)), P);
   std::vector<int> table((size_t)stride * std::max(E, 1), P), tileStart((size_t)(tiles + 1) * std::max(E, 1), P);
-  std::vector<float4> xyw(N, float4{NAN, NAN, NAN, NAN}), xyrp(N, float4{NAN, NAN, NAN, NAN});
+  std::vector<float4> xyw(N, float4{NAN, NAN, NAN, NAN});
+  std::vector<float> rhoS(N, NAN);
   std::vector<float> rho(N, -7.f), delta(N, -7.f);
   std::vector<int> nh(N, -7), isSeed(N, -7), ci(N, -7);
   int err = 0;
@@ -71,21 +72,21 @@ int main(int argc, char** argv) {
     emu::launch(nb, kThreads, [&] {
       k_radix_scatter<KeyArray, DigitL, EmitLast2D>(
           blocks.data(), KeyArray{k2.data()}, hist.data(),
-          EmitLast2D{i2.data(), k1.data(), i1.data(), in.x.data(), in.y.data(), in.w.data(), xyw.data()});
+          EmitLast2D{i2.data(), k1.data(), in.x.data(), in.y.data(), in.w.data(), xyw.data()});
     });
     build_table(k1.data(), blocks.data(), nb, evOff.data(), E, k2dKeys, stride, table.data(), tileStart.data());
     const int dv = argc > 3 ? atoi(argv[3]) : 0, nv = argc > 4 ? atoi(argv[4]) : 0;
     dispatch(dv, [&](auto I, auto S) {
       emu::launch(CompShape<I>::kPerSort * nb, kThreads, [&] {
-        k_density2d<I, S>(blocks.data(), k1.data(), xyw.data(), i1.data(), table.data(), stride, dc, dc2,
-                          xyrp.data(), rho.data());
+        k_density2d<I, S>(blocks.data(), k1.data(), xyw.data(), table.data(), stride, dc, dc2, rhoS.data(),
+                          rho.data());
       });
     });
     int* parent = reinterpret_cast<int*>(k2.data());
     dispatch(nv, [&](auto I, auto S) {
       emu::launch(CompShape<I>::kPerSort * nb, kThreads, [&] {
-        k_nh2d<I, S>(blocks.data(), k1.data(), xyrp.data(), table.data(), stride, evOff.data(), dc, rhoc, odf, dm2,
-                     delta.data(), nh.data(), isSeed.data(), parent);
+        k_nh2d<I, S>(blocks.data(), k1.data(), xyw.data(), rhoS.data(), table.data(), stride, evOff.data(), dc, rhoc,
+                     odf, dm2, delta.data(), nh.data(), isSeed.data(), parent);
       });
     });
     SeedRankHost seeds(N);
