@@ -11,6 +11,7 @@
 #include <barrier>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <memory>
@@ -22,7 +23,12 @@
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
-#define __shared__ static
+// Every `__shared__` array lives in one linker section, which launch() fills with a poison byte
+// before each workgroup: LDS starts with arbitrary contents on the device, so code that reads LDS
+// before writing it must not pass here by reading zeros or a previous workgroup's values.
+#define __shared__ static __attribute__((section("emu_lds")))
+extern "C" char __start_emu_lds[];
+extern "C" char __stop_emu_lds[];
 #define __align__(n) __attribute__((aligned(n)))
 #define __builtin_amdgcn_sched_barrier(x) ((void)0)
 
@@ -81,6 +87,15 @@ inline Group* group = nullptr;
 
 inline Wave& my_wave() { return group->waves[tIdx.x / kWave]; }
 
+// LDS poison byte (environment EMU_LDS_POISON, default 0x5A).
+inline int lds_poison() {
+  static const int v = [] {
+    const char* e = std::getenv("EMU_LDS_POISON");
+    return e ? (int)std::strtol(e, nullptr, 0) & 0xff : 0x5A;
+  }();
+  return v;
+}
+
 // Run body() over grid x block threads, one workgroup at a time; the block's threads are created
 // once per launch and step through the workgroups together.
 template <class F>
@@ -114,6 +129,7 @@ void launch(unsigned grid, unsigned block, F&& body) {
       g.waves[w].bar = std::make_unique<std::barrier<>>(lanes);
     }
     group = &g;
+    std::memset(__start_emu_lds, lds_poison(), (size_t)(__stop_emu_lds - __start_emu_lds));
     start.arrive_and_wait();
     done.arrive_and_wait();
     group = nullptr;
