@@ -98,9 +98,17 @@ def cpu_baseline(packed, offs, p, gpu_out):
     same = all(np.array_equal(np.ascontiguousarray(out[k]).view(np.int32),
                               np.ascontiguousarray(gpu_out[k]).view(np.int32)) for k in out)
     n = int(offs[-1])
+    # one thread on one event: the 1e5-hit (100 layers x 1e3) and the 1e6-hit (100 x 1e4) events
+    single = {}
+    for name, hits in (("1e5_hits", 100_000), ("1e6_hits", 1_000_000)):
+        ev, eo = synth.pack([synth.clue2d_event(hits, 0)])
+        t1 = time.perf_counter()
+        R.clue2d_batch(ev, eo, p.dc, p.rhoc, p.outlierDeltaFactor, **({"threads": 1} if kind == "reference" else {}))
+        single[name] = round(hits / (time.perf_counter() - t1) / 1e6, 3)
     return {"value": round(n / t / 1e6, 3), "unit": "Mhits/s", "cores": threads, "kind": kind,
             "sample": f"all {offs.size - 1} events of the workload ({n} hits), one event per thread at a time",
-            "seconds": round(t, 3), "labels_bit_exact_vs_gpu": bool(same)}
+            "seconds": round(t, 3), "labels_bit_exact_vs_gpu": bool(same),
+            "single_thread_one_event_mhits_per_s": single}
 
 
 def per_event_run3d(dev, din, dout, offs, p):
@@ -162,6 +170,67 @@ def clue3d_leg(dev, n_events, steps, warmup):
                         "algorithmic_bytes_per_launch": algo}}
     log(f"clue3d: {ms:.3f} ms/step ({value:.1f} Mclusters/s), per-event {t * 1e3:.1f} ms, bit-exact={same}")
     return out
+
+
+def pcie_leg(dev, packed, din, dout, plan, N, steps, batch_out):
+    """The batched step with its host transfers: pinned host -> device copies of the inputs, the graph,
+    device -> host copies of the outputs, all on the launch stream (PCIe-inclusive rate)."""
+    lib, ctx = dev.lib, dev.ctx
+    hin = {k: dev.pinned_copy(np.ascontiguousarray(packed[k], dtype=din[k].dtype)) for k in clue_hip.FIELDS_2D_IN}
+    hout = {k: dev.pinned_copy(np.zeros(N, dtype=dout[k].dtype)) for k in clue_hip.FIELDS_2D_OUT}
+    dev.sync()
+    lib.clue_mark(ctx, 4, None)
+    for _ in range(steps):
+        for k in clue_hip.FIELDS_2D_IN:
+            dev._chk(lib.clue_memcpy_htod(ctx, din[k].ptr, hin[k].ctypes.data, hin[k].nbytes, None))
+        plan.launch()
+        for k in clue_hip.FIELDS_2D_OUT:
+            dev._chk(lib.clue_memcpy_dtoh(ctx, hout[k].ctypes.data, dout[k].ptr, hout[k].nbytes, None))
+    lib.clue_mark(ctx, 5, None)
+    dev.sync()
+    dev.check()
+    ms = lib.clue_elapsed_ms(ctx, 4, 5) / steps
+    same = all(np.array_equal(hout[k].view(np.int32), batch_out[k].view(np.int32)) for k in hout)
+    moved = sum(a.nbytes for a in hin.values()) + sum(a.nbytes for a in hout.values())
+    log(f"pcie-inclusive: {ms:.3f} ms/step ({N / ms / 1e3:.1f} Mhits/s, {moved / ms / 1e6:.1f} GB/s over PCIe)")
+    return {"value": round(N / (ms * 1e-3) / 1e6, 2), "unit": "Mhits/s", "ms_per_step": round(ms, 4),
+            "bytes_over_pcie_per_step": moved, "steps": steps, "labels_bit_exact_vs_batched": bool(same)}
+
+
+def c2_run(args):
+    """One event of 1e6 hits (100 layers x 1e4) as a single-event plan: the configuration the per-kernel
+    counter profiles of a single large event are taken on."""
+    ev, offs = synth.pack([synth.clue2d_event(1_000_000, 0)])
+    N = int(offs[-1])
+    dev = clue_hip.Device(0)
+    p = clue_hip.params2d(3.0, 10.0, 2.0)
+    din = dev.upload_soa(ev, clue_hip.FIELDS_2D_IN)
+    dout = dev.alloc_soa(N, clue_hip.FIELDS_2D_OUT)
+    plan = dev.clue2d_plan(din, dout, offs, p, use_graph=True, autotune=True)
+    for _ in range(max(2, args.warmup)):
+        plan.launch()
+    dev.check()
+    lib, ctx = dev.lib, dev.ctx
+    dev.sync()
+    lib.clue_mark(ctx, 0, None)
+    for _ in range(args.steps):
+        plan.launch()
+    lib.clue_mark(ctx, 1, None)
+    dev.sync()
+    dev.check()
+    ms = lib.clue_elapsed_ms(ctx, 0, 1) / args.steps
+    algo = BYTES_PER_HIT_2D * N
+    out = {"metric": "Mhits/s, one 1e6-hit event", "value": round(N / (ms * 1e-3) / 1e6, 2), "unit": "Mhits/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+           "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "clue2d_single_event_1e6_hits_100_layers", "hits": N, "launch": "hipGraph",
+                      "autotuned": plan.tuning},
+           "roofline": {"bound": "hbm", "achieved": round(algo / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "algorithmic_bytes_per_launch": algo}}
+    plan.destroy()
+    dev.close()
+    print(json.dumps(out), flush=True)
 
 
 def serving_run(dev, pool, order, arrivals, slots, max_events, p3):
@@ -240,6 +309,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="serving leg: length of the stream")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-3d", action="store_true", help="skip the CLUE3D (512-event) leg")
+    ap.add_argument("--workload", choices=("c3", "c2"), default="c3",
+                    help="c3: the 256-event batch (default, the headline line); c2: only one 1e6-hit event")
+    ap.add_argument("--no-pcie", action="store_true", help="skip the PCIe-inclusive figure")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run 3 device-to-device copies of BYTES (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -250,6 +322,9 @@ def main():
 
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
+        return
+    if args.workload == "c2":
+        c2_run(args)
         return
 
     counts = synth.c3_hit_counts(args.events)
@@ -301,6 +376,9 @@ def main():
         result["roofline"]["traffic_source"] = str(TRAFFIC_PROFILE.relative_to(REPO))
         result["roofline"]["traffic_over_algorithmic"] = round(traffic / algo_bytes, 2)
         result["roofline"]["hbm_gbs_measured_traffic"] = round(traffic / (ms * 1e-3) / 1e9, 1)
+
+    if not args.no_pcie:
+        result["pcie_inclusive"] = pcie_leg(dev, packed, din, dout, plan, N, min(args.steps, 5), batch_out)
 
     if not args.no_per_event:
         pout = dev.alloc_soa(N, clue_hip.FIELDS_2D_OUT)
