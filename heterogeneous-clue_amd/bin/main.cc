@@ -3,9 +3,10 @@
 //
 // Each EDM stream is a std::thread owning its own module instances (as StreamSchedule builds its
 // own workers, src/serial/bin/StreamSchedule.cc:24-38); streams pull events from a shared source
-// until maxEvents are done (cycling over the file's events like Source2D/3D::produce,
-// src/serial/bin/Source.cc:152-227). Modules are found by name in plugin libraries loaded with
-// dlopen (src/serial/bin/PluginManager.cc, SharedLibrary.cc).
+// until maxEvents are done, or until runForMinutes have passed at the end of a pass over the file
+// (cycling over the file's events like Source2D/3D::produce, src/serial/bin/Source.cc:152-227).
+// Modules are found by name in plugin libraries loaded with dlopen (src/serial/bin/PluginManager.cc,
+// SharedLibrary.cc). --validation adds the CLUE2D validator (src/serial/bin/main.cc:80-85, 157-168).
 #include <dlfcn.h>
 #include <time.h>
 
@@ -13,6 +14,8 @@
 #include <chrono>
 #include <cstdint>
 #include <cstdlib>
+#include <cstring>
+#include <iterator>
 #include <exception>
 #include <filesystem>
 #include <fstream>
@@ -25,6 +28,7 @@
 #include <thread>
 #include <vector>
 
+#include "DataFormats/CLUEValidatorTypes.h"
 #include "DataFormats/CLUE_config.h"
 #include "DataFormats/ClusterCollection.h"
 #include "DataFormats/PointsCloud.h"
@@ -71,6 +75,27 @@ namespace {
     return evs;
   }
 
+  // A toyDetector file is one event with no header: records {x, y, layer, weight} (f32), repeated on
+  // each of the 100 layers with layer + l (Source.cc:60-79); a trailing partial record is ignored.
+  std::vector<PointsCloud> readToyDetector(std::filesystem::path const& f) {
+    std::ifstream in(f, std::ios::binary);
+    if (!in) throw std::runtime_error("cannot open " + f.string());
+    const std::vector<char> bytes{std::istreambuf_iterator<char>(in), std::istreambuf_iterator<char>()};
+    const size_t n = bytes.size() / (4 * sizeof(float));
+    std::vector<float> rec(n * 4);
+    std::memcpy(rec.data(), bytes.data(), rec.size() * sizeof(float));
+    PointsCloud pc;
+    constexpr int kLayers = 100;
+    for (int l = 0; l < kLayers; ++l)
+      for (size_t i = 0; i < n; ++i) {
+        pc.x.push_back(rec[4 * i]);
+        pc.y.push_back(rec[4 * i + 1]);
+        pc.layer.push_back(static_cast<int>(rec[4 * i + 2] + static_cast<float>(l)));
+        pc.weight.push_back(rec[4 * i + 3]);
+      }
+    return {std::move(pc)};
+  }
+
   std::vector<ClusterCollection> read3D(std::filesystem::path const& f) {
     std::ifstream in(f, std::ios::binary);
     if (!in) throw std::runtime_error("cannot open " + f.string());
@@ -97,6 +122,43 @@ namespace {
     return evs;
   }
 
+  // ------------------------------------------------------------ event numbering (Source.cc:152-190)
+  // Hands out 0-based event numbers: the first maxEvents, or, with runForMinutes >= 0, whole passes
+  // over the file's events until that many minutes have passed when a pass ends.
+  class EventCounter {
+  public:
+    EventCounter(int maxEvents, int runForMinutes, int eventsInFile)
+        : maxEvents_(maxEvents), minutes_(runForMinutes), cycle_(eventsInFile), start_(std::chrono::steady_clock::now()) {}
+
+    int take() {
+      if (stop_.load()) return -1;
+      const int k = next_.fetch_add(1);
+      if (minutes_ < 0) {
+        if (k < maxEvents_) return k;
+        stop_.store(true);
+        return -1;
+      }
+      if (k - lastCheck_.load() >= cycle_) {
+        std::lock_guard<std::mutex> g(m_);
+        if (k - lastCheck_.load() >= cycle_) {
+          const auto elapsed = std::chrono::steady_clock::now() - start_;
+          if (std::chrono::duration_cast<std::chrono::minutes>(elapsed).count() >= minutes_) stop_.store(true);
+          lastCheck_.store((k / cycle_) * cycle_);
+        }
+      }
+      return stop_.load() ? -1 : k;
+    }
+
+    void stop() { stop_.store(true); }
+
+  private:
+    const int maxEvents_, minutes_, cycle_;
+    const std::chrono::steady_clock::time_point start_;
+    std::atomic<int> next_{0}, lastCheck_{0};
+    std::atomic<bool> stop_{false};
+    std::mutex m_;
+  };
+
   // ------------------------------------------------------------ one EDM stream
   template <typename Input>
   class Stream {
@@ -110,11 +172,9 @@ namespace {
       }
     }
 
-    void run(std::vector<Input> const& events, edm::EventSetup const& es, std::atomic<int>& next, int maxEvents,
+    void run(std::vector<Input> const& events, edm::EventSetup const& es, EventCounter& counter,
              std::atomic<int>& processed) {
-      for (;;) {
-        const int k = next.fetch_add(1);
-        if (k >= maxEvents) return;
+      for (int k; (k = counter.take()) >= 0;) {
         edm::Event ev(id_, k + 1, reg_);
         ev.emplace(token_, events[static_cast<size_t>(k) % events.size()]);
         for (auto& w : workers_) w->doWork(ev, es);
@@ -135,10 +195,11 @@ namespace {
 
   template <typename Input>
   int process(std::vector<Input> const& events, std::vector<std::string> const& modules, edm::EventSetup const& es,
-              int numberOfStreams, int maxEvents, double& seconds) {
+              int numberOfStreams, int maxEvents, int runForMinutes, double& seconds) {
     std::vector<std::unique_ptr<Stream<Input>>> streams;
     for (int s = 0; s < numberOfStreams; ++s) streams.push_back(std::make_unique<Stream<Input>>(s, modules));
-    std::atomic<int> next{0}, processed{0};
+    EventCounter counter(maxEvents, runForMinutes, static_cast<int>(events.size()));
+    std::atomic<int> processed{0};
     std::exception_ptr err;
     std::mutex m;
     const auto t0 = std::chrono::steady_clock::now();
@@ -146,11 +207,11 @@ namespace {
     for (auto& s : streams)
       threads.emplace_back([&, sp = s.get()] {
         try {
-          sp->run(events, es, next, maxEvents, processed);
+          sp->run(events, es, counter, processed);
         } catch (...) {
           std::lock_guard<std::mutex> g(m);
           if (!err) err = std::current_exception();
-          next.store(maxEvents);  // stop the other streams
+          counter.stop();  // stop the other streams
         }
       });
     for (auto& t : threads) t.join();
@@ -170,15 +231,26 @@ namespace {
     std::cout
         << name << ": CLUE / CLUE3D on the HIP backend\n\n"
         << "Usage: " << name
-        << " [--dim 2|3] [--numberOfThreads NT] [--numberOfStreams NS] [--maxEvents ME] [--inputFile PATH]\n"
-        << "       [--configFile PATH] [--outputDir DIR] [--dumpDir DIR] [--plugin LIB] [--empty]\n\n"
+        << " [--dim 2|3] [--numberOfThreads NT] [--numberOfStreams NS] [--maxEvents ME | --runForMinutes MIN]\n"
+        << "       [--inputFile PATH] [--configFile PATH] [--validation] [--strictValidation] [--outputDir DIR]\n"
+        << "       [--dumpDir DIR] [--plugin LIB] [--empty]\n\n"
         << "  --dim               2 (CLUE, default) or 3 (CLUE3D)\n"
         << "  --numberOfThreads   threads (default 1); each runs one EDM stream\n"
         << "  --numberOfStreams   concurrent events (default: number of threads)\n"
-        << "  --maxEvents         events to process (default: all events of the input file)\n"
+        << "  --maxEvents         events to process (default: all events of the input file; 10 for a\n"
+        << "                      toyDetector file)\n"
+        << "  --runForMinutes     process passes over the input file until this many minutes have passed\n"
         << "  --inputFile         raw input (2D: uint32 n + n x {x,y,layer,weight} f32 per event;\n"
-        << "                      3D: n x {x,y,z,eta,phi,r_over_absz,radius,layer,energy,isSilicon})\n"
-        << "  --configFile        2D: CSV dc,rhoc,outlierDeltaFactor,produceOutput; 3D: name,value lines\n"
+        << "                      3D: n x {x,y,z,eta,phi,r_over_absz,radius,layer,energy,isSilicon});\n"
+        << "                      a 2D file named toyDetector* holds one layer's {x,y,layer,weight} records\n"
+        << "                      with no header, repeated on all 100 layers (default: data/input/raw2D.bin\n"
+        << "                      or raw3D.bin next to this program)\n"
+        << "  --configFile        2D: CSV dc,rhoc,outlierDeltaFactor,produceOutput (default:\n"
+        << "                      config/hgcal_config.csv next to this program); 3D: name,value lines\n"
+        << "  --validation        check each event against <input dir>/../output/reference/ref_<N>k_20_25_2.csv\n"
+        << "                      for a toyDetector_<N>k input (CLUE 2D only); after --inputFile it selects\n"
+        << "                      config/toyDetector_config.csv next to this program\n"
+        << "  --strictValidation  --validation, and every result row must print exactly as in the file\n"
         << "  --outputDir         where CLUEOutputProducer writes its CSV files (2D, produceOutput = 1)\n"
         << "  --dumpDir           write bit-exact binary outputs per event (Event_<id>.bin)\n"
         << "  --plugin            plugin library to load (default: libclue_plugins.so next to this program)\n"
@@ -189,10 +261,11 @@ namespace {
 
 int main(int argc, char** argv) {
   std::vector<std::string> args(argv, argv + argc);
-  int dim = 2, numberOfThreads = 1, numberOfStreams = 0, maxEvents = -1;
+  int dim = 2, numberOfThreads = 1, numberOfStreams = 0, maxEvents = -1, runForMinutes = -1;
   std::filesystem::path inputFile, configFile, outputDir, dumpDir;
   std::vector<std::string> plugins;
-  bool empty = false;
+  bool empty = false, validation = false, strictValidation = false;
+  const std::filesystem::path exeDir = std::filesystem::path(args[0]).parent_path();
   try {
     for (size_t i = 1; i < args.size(); ++i) {
       auto next = [&]() -> std::string const& {
@@ -210,10 +283,17 @@ int main(int argc, char** argv) {
         numberOfStreams = std::stoi(next());
       } else if (args[i] == "--maxEvents") {
         maxEvents = std::stoi(next());
+      } else if (args[i] == "--runForMinutes") {
+        runForMinutes = std::stoi(next());
       } else if (args[i] == "--inputFile") {
         inputFile = next();
       } else if (args[i] == "--configFile") {
         configFile = next();
+      } else if (args[i] == "--validation" || args[i] == "--strictValidation") {
+        validation = true;
+        strictValidation = strictValidation || args[i] == "--strictValidation";
+        if (inputFile.string().find("toyDetector") != std::string::npos)
+          configFile = exeDir / "config" / "toyDetector_config.csv";
       } else if (args[i] == "--outputDir") {
         outputDir = next();
       } else if (args[i] == "--dumpDir") {
@@ -236,13 +316,19 @@ int main(int argc, char** argv) {
     std::cout << "The dimensionality of the algorithm is either 2 or 3!" << std::endl;
     return EXIT_FAILURE;
   }
+  if (maxEvents >= 0 && runForMinutes >= 0) {
+    std::cout << "Got both --maxEvents and --runForMinutes, please give only one of them" << std::endl;
+    return EXIT_FAILURE;
+  }
   if (numberOfThreads <= 0) numberOfThreads = static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
   if (numberOfStreams <= 0) numberOfStreams = numberOfThreads;
-  if (inputFile.empty() || !std::filesystem::exists(inputFile)) {
+  if (inputFile.empty()) inputFile = exeDir / "data" / "input" / (dim == 2 ? "raw2D.bin" : "raw3D.bin");
+  if (configFile.empty() && dim == 2) configFile = exeDir / "config" / "hgcal_config.csv";
+  if (!std::filesystem::exists(inputFile)) {
     std::cout << "Input file '" << inputFile.string() << "' does not exist" << std::endl;
     return EXIT_FAILURE;
   }
-  if (dim == 2 && (configFile.empty() || !std::filesystem::exists(configFile))) {
+  if (dim == 2 && !std::filesystem::exists(configFile)) {
     std::cout << "Config file '" << configFile.string() << "' does not exist" << std::endl;
     return EXIT_FAILURE;
   }
@@ -260,6 +346,14 @@ int main(int argc, char** argv) {
     double seconds = 0;
     int processed = 0;
     const double cpu0 = cpu_seconds();
+    auto announce = [&](int events) {
+      if (runForMinutes < 0)
+        std::cout << "Processing " << events << " events, of which " << numberOfStreams << " concurrently, with "
+                  << numberOfThreads << " threads." << std::endl;
+      else
+        std::cout << "Processing for about " << runForMinutes << " minutes with " << numberOfStreams
+                  << " concurrent events and " << numberOfThreads << " threads." << std::endl;
+    };
     if (dim == 2) {
       edm::ESPluginFactory::create("CLUEHIPClusterizerESProducer", configFile)->produce(es);
       Parameters const& par = es.get<Parameters>();
@@ -267,23 +361,29 @@ int main(int argc, char** argv) {
                 << "dc = " << par.dc << '\n'
                 << "rhoc = " << par.rhoc << '\n'
                 << "outlierDeltaFactor = " << par.outlierDeltaFactor << std::endl;
+      if (par.produceOutput) std::cerr << "Producing output at the end" << std::endl;
       if (!empty) {
         modules.push_back("CLUEHIPClusterizer");
         if (par.produceOutput) {
           edm::ESPluginFactory::create("CLUEOutputESProducer", outputDir.empty() ? "output" : outputDir)->produce(es);
           modules.push_back("CLUEOutputProducer");
         }
+        if (validation) {
+          edm::ESPluginFactory::create("CLUEValidatorESProducer", inputFile)->produce(es);
+          es.put(std::make_unique<ValidationMode>(ValidationMode{strictValidation}));
+          modules.push_back("CLUEValidator");
+        }
         if (!dumpDir.empty()) {
           edm::ESPluginFactory::create("CLUEDumpESProducer", dumpDir)->produce(es);
           modules.push_back("CLUEBinaryDump");
         }
       }
-      auto events = read2D(inputFile);
+      const bool toy = inputFile.string().find("toyDetector") != std::string::npos;
+      auto events = toy ? readToyDetector(inputFile) : read2D(inputFile);
       if (events.empty()) throw std::runtime_error("input file holds no events");
-      if (maxEvents < 0) maxEvents = static_cast<int>(events.size());
-      std::cout << "Processing " << maxEvents << " events, of which " << numberOfStreams << " concurrently, with "
-                << numberOfThreads << " threads." << std::endl;
-      processed = process(events, modules, es, numberOfStreams, maxEvents, seconds);
+      if (maxEvents < 0 && runForMinutes < 0) maxEvents = toy ? 10 : static_cast<int>(events.size());
+      announce(maxEvents);
+      processed = process(events, modules, es, numberOfStreams, maxEvents, runForMinutes, seconds);
     } else {
       if (!configFile.empty()) {
         edm::ESPluginFactory::create("CLUEHIPTracksterizerESProducer", configFile)->produce(es);
@@ -293,6 +393,7 @@ int main(int argc, char** argv) {
       }
       if (!empty) {
         modules.push_back("CLUEHIPTracksterizer");
+        if (validation) std::cerr << "Validation not available for CLUE 3D" << std::endl;
         if (!dumpDir.empty()) {
           edm::ESPluginFactory::create("CLUEDumpESProducer", dumpDir)->produce(es);
           modules.push_back("CLUE3DBinaryDump");
@@ -300,10 +401,9 @@ int main(int argc, char** argv) {
       }
       auto events = read3D(inputFile);
       if (events.empty()) throw std::runtime_error("input file holds no events");
-      if (maxEvents < 0) maxEvents = static_cast<int>(events.size());
-      std::cout << "Processing " << maxEvents << " events, of which " << numberOfStreams << " concurrently, with "
-                << numberOfThreads << " threads." << std::endl;
-      processed = process(events, modules, es, numberOfStreams, maxEvents, seconds);
+      if (maxEvents < 0 && runForMinutes < 0) maxEvents = static_cast<int>(events.size());
+      announce(maxEvents);
+      processed = process(events, modules, es, numberOfStreams, maxEvents, runForMinutes, seconds);
     }
     const double cpu = cpu_seconds() - cpu0;
     std::cout << "Processed " << processed << " events in " << std::scientific << seconds << " seconds, throughput "

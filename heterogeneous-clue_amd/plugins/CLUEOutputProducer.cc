@@ -5,6 +5,7 @@
 #include <fstream>
 #include <iostream>
 
+#include "DataFormats/CLUECsvRow.h"
 #include "DataFormats/CLUE_config.h"
 #include "DataFormats/PointsCloud.h"
 #include "Framework/EDProducer.h"
@@ -24,11 +25,10 @@ private:
     auto const& outDir = eventSetup.get<std::filesystem::path>();
     const auto outFile = outDir / create_outputfileName(event.eventID(), par.dc, par.rhoc, par.outlierDeltaFactor);
     std::ofstream out(outFile);
-    out << "index,x,y,layer,weight,rho,delta,nh,isSeed,clusterId\n";
+    out << kClue2DCsvHeader << "\n";
     for (size_t i = 0; i < results.x.size(); ++i) {
-      out << i << "," << results.x[i] << "," << results.y[i] << "," << results.layer[i] << "," << results.weight[i]
-          << "," << results.rho[i] << "," << (results.delta[i] > 999 ? 999 : results.delta[i]) << ","
-          << results.nearestHigher[i] << "," << results.isSeed[i] << "," << results.clusterIndex[i] << "\n";
+      write_clue2d_row(out, results, i);
+      out << "\n";
     }
     std::cout << "Ouput was saved in " << outFile << std::endl;
   }

@@ -171,6 +171,32 @@ def write_raw(path, events: list[dict], dim: int = 2) -> None:
             f.write(rec.tobytes())
 
 
+def toy_detector_layer(n_points: int, seed: int = SEED_BASE + 7) -> dict:
+    """One layer of a toy-detector event sized for dc = 20, rhoc = 25: the CLUE2D model on layer 0
+    with blobs of Poisson(40) hits, sigma 5 cm, 80 % of the hits in blobs."""
+    return clue2d_event(n_points, 0, seed, n_layers=1, blob_frac=0.8, blob_mean=40.0, sigma=5.0)
+
+
+def write_toy_detector(path, layer_event: dict) -> None:
+    """A toyDetector file: the records {x, y, layer, weight} (float32) of one layer, no header
+    (the reference reads it in src/serial/bin/Source.cc:60-79)."""
+    rec = np.stack([layer_event[k].astype(np.float32) for k in _FIELDS_2D], axis=1)
+    with open(path, "wb") as f:
+        f.write(np.ascontiguousarray(rec).tobytes())
+
+
+def expand_toy_detector(layer_event: dict, n_layers: int = NLAYERS_2D) -> dict:
+    """The event a toyDetector file stands for: its records repeated on layers 0..99, layer
+    int(float32(layer) + l) for copy l."""
+    lay = layer_event["layer"].astype(np.float32)
+    return {
+        "x": np.tile(layer_event["x"].astype(np.float32), n_layers),
+        "y": np.tile(layer_event["y"].astype(np.float32), n_layers),
+        "layer": np.concatenate([(lay + np.float32(l)).astype(np.int32) for l in range(n_layers)]),
+        "weight": np.tile(layer_event["weight"].astype(np.float32), n_layers),
+    }
+
+
 def read_raw(path, dim: int = 2) -> list[dict]:
     fields = _FIELDS_2D if dim == 2 else _FIELDS_3D
     data = np.fromfile(path, dtype=np.uint8)

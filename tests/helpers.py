@@ -88,3 +88,23 @@ def check_invariants_3d(ev: dict, out: dict):
     assert np.all(nhl[has] != lay[i])
     s = np.nonzero(seed)[0]
     assert np.array_equal(ti[s], np.arange(s.size, dtype=np.int32))
+
+
+CSV_HEADER_2D = "index,x,y,layer,weight,rho,delta,nh,isSeed,clusterId"
+
+
+def _g(v) -> str:
+    """A float32 as C++ iostreams print it by default (printf %g, 6 significant digits)."""
+    return "%g" % float(np.float32(v))
+
+
+def clue2d_csv_lines(ev: dict, out: dict) -> list[str]:
+    """The result CSV of one CLUE2D event, in the reference writer's format: header, then one row per
+    point with delta clamped at 999."""
+    rows = [CSV_HEADER_2D]
+    for i in range(ev["x"].size):
+        d = out["delta"][i]
+        rows.append(",".join([str(i), _g(ev["x"][i]), _g(ev["y"][i]), str(int(ev["layer"][i])), _g(ev["weight"][i]),
+                              _g(out["rho"][i]), _g(999 if d > 999 else d), str(int(out["nearestHigher"][i])),
+                              str(int(out["isSeed"][i])), str(int(out["clusterIndex"][i]))]))
+    return rows

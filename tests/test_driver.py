@@ -30,6 +30,16 @@ def test_driver_help_and_bad_arguments():
     assert r.returncode != 0 and "either 2 or 3" in r.stdout
     r = run(["--inputFile", "/nonexistent.bin"])
     assert r.returncode != 0 and "does not exist" in r.stdout
+    r = run(["--maxEvents", "3", "--runForMinutes", "1"])
+    assert r.returncode != 0 and "please give only one of them" in r.stdout
+    # without --inputFile the driver looks for data/input/raw2D.bin next to itself
+    r = run([])
+    assert r.returncode != 0 and "data/input/raw2D.bin' does not exist" in r.stdout
+
+
+def test_driver_default_configuration_files():
+    assert (PKG / "config" / "hgcal_config.csv").read_text().strip() == "1.3,0.11,2,0"
+    assert (PKG / "config" / "toyDetector_config.csv").read_text().strip() == "20,25,2,0"
 
 
 def test_driver_truncated_input_and_missing_device(tmp_path):
@@ -37,7 +47,7 @@ def test_driver_truncated_input_and_missing_device(tmp_path):
     synth.write_raw(f, [synth.clue2d_event(500, 1)], dim=2)
     cfg = tmp_path / "cfg.csv"
     cfg.write_text("3,10,2,0\n")
-    r = run(["--dim", "2", "--inputFile", str(f)])
+    r = run(["--dim", "2", "--inputFile", str(f), "--configFile", str(tmp_path / "absent.csv")])
     assert r.returncode != 0 and "Config file" in r.stdout
     with open(f, "ab") as h:
         h.write(b"\x10\x00\x00\x00\x01\x02")

@@ -81,3 +81,74 @@ def test_driver_reports_bad_layer(tmp_path):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode != 0
     assert "Caught std::runtime_error" in r.stdout and "layer" in r.stdout
+
+
+def toy_layout(root, lines):
+    """data/input/toyDetector_1k.bin (written by the caller) and the reference result file the
+    validator reads for it, data/output/reference/ref_1k_20_25_2.csv."""
+    (root / "data" / "input").mkdir(parents=True)
+    ref = root / "data" / "output" / "reference"
+    ref.mkdir(parents=True)
+    (ref / "ref_1k_20_25_2.csv").write_text("\n".join(lines) + "\n")
+    return root / "data" / "input" / "toyDetector_1k.bin"
+
+
+def test_driver_toy_detector_validation(tmp_path, oracle_lib):
+    layer = synth.toy_detector_layer(1000)
+    ev = synth.expand_toy_detector(layer)
+    want = oracle_lib.clue2d(ev["x"], ev["y"], ev["layer"], ev["weight"], 20.0, 25.0, 2.0)
+    inp = toy_layout(tmp_path, helpers.clue2d_csv_lines(ev, want))
+    synth.write_toy_detector(inp, layer)
+    main = str(PKG / "clue_hip_main")
+    print("case driver toyDetector --validation", flush=True)
+    # --validation after --inputFile selects config/toyDetector_config.csv (20, 25, 2); 10 events by default
+    r = subprocess.run([main, "--inputFile", str(inp), "--validation", "--numberOfThreads", "2",
+                        "--dumpDir", str(tmp_path / "dump")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Processing 10 events" in r.stdout and "dc = 20" in r.stderr
+    assert r.stdout.count("CLUE output is correct!") == 10
+    for eid in (1, 10):
+        got = read_dump(tmp_path / "dump" / f"Event_{eid}.bin", 2)
+        helpers.assert_bit_exact(got, want, helpers.OUT2D, f"toyDetector event {eid}")
+    print("case driver toyDetector --strictValidation", flush=True)
+    r = subprocess.run([main, "--inputFile", str(inp), "--strictValidation", "--maxEvents", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("identical to the reference file") == 2
+    print("case driver toyDetector --runForMinutes 0", flush=True)
+    r = subprocess.run([main, "--inputFile", str(inp), "--validation", "--runForMinutes", "0"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Processing for about 0 minutes" in r.stdout and "Processed 1 events" in r.stdout
+    print("case driver toyDetector other parameters", flush=True)
+    cfg = tmp_path / "cfg.csv"
+    cfg.write_text("3,10,2,0\n")
+    r = subprocess.run([main, "--inputFile", str(inp), "--validation", "--configFile", str(cfg), "--maxEvents", "1"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "cannot check results" in r.stdout, r.stdout + r.stderr
+
+
+def test_driver_validation_failure_and_3d(tmp_path, oracle_lib):
+    layer = synth.toy_detector_layer(600, seed=11)
+    ev = synth.expand_toy_detector(layer)
+    want = oracle_lib.clue2d(ev["x"], ev["y"], ev["layer"], ev["weight"], 20.0, 25.0, 2.0)
+    lines = helpers.clue2d_csv_lines(ev, want)
+    k = int(np.flatnonzero(want["rho"] > 1)[3])
+    f = lines[k + 1].split(",")
+    f[5] = "%g" % (float(f[5]) + 0.01)
+    lines[k + 1] = ",".join(f)
+    inp = toy_layout(tmp_path, lines)
+    synth.write_toy_detector(inp, layer)
+    print("case driver toyDetector corrupted reference", flush=True)
+    r = subprocess.run([str(PKG / "clue_hip_main"), "--inputFile", str(inp), "--validation", "--maxEvents", "1"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "Caught std::runtime_error" in r.stdout and f"rho differs from the reference at i={k}," in r.stdout
+    evs = [synth.clue3d_event(5)]
+    f3 = tmp_path / "raw3D.bin"
+    synth.write_raw(f3, evs, dim=3)
+    print("case driver clue3d --validation", flush=True)
+    r = subprocess.run([str(PKG / "clue_hip_main"), "--dim", "3", "--inputFile", str(f3), "--validation"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Validation not available for CLUE 3D" in r.stderr
