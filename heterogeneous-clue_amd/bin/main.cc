@@ -1,10 +1,11 @@
 // clue_hip_main: runs the CLUE2D / CLUE3D HIP modules over the events of an input file, with the
 // command line and final report of the reference's driver (src/serial/bin/main.cc:43-240).
 //
-// Each EDM stream is a std::thread owning its own module instances (as StreamSchedule builds its
-// own workers, src/serial/bin/StreamSchedule.cc:24-38); streams pull events from a shared source
-// until maxEvents are done, or until runForMinutes have passed at the end of a pass over the file
-// (cycling over the file's events like Source2D/3D::produce, src/serial/bin/Source.cc:152-227).
+// numberOfStreams EDM streams, each owning its own module instances, run as tasks on a pool of
+// numberOfThreads threads (edm::StreamSchedule, edm::TaskPool); an external-work module's device
+// work frees the thread for other streams until it completes. Streams pull events from a shared
+// source until maxEvents are done, or until runForMinutes have passed at the end of a pass over the
+// file (cycling over the file's events like Source2D/3D::produce, src/serial/bin/Source.cc:152-227).
 // Modules are found by name in plugin libraries loaded with dlopen (src/serial/bin/PluginManager.cc,
 // SharedLibrary.cc). --validation adds the CLUE2D validator (src/serial/bin/main.cc:80-85, 157-168).
 #include <dlfcn.h>
@@ -12,6 +13,7 @@
 
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -37,7 +39,8 @@
 #include "Framework/EventSetup.h"
 #include "Framework/PluginFactory.h"
 #include "Framework/ProductRegistry.h"
-#include "Framework/Worker.h"
+#include "Framework/StreamSchedule.h"
+#include "Framework/TaskPool.h"
 
 namespace {
 
@@ -159,62 +162,36 @@ namespace {
     std::mutex m_;
   };
 
-  // ------------------------------------------------------------ one EDM stream
-  template <typename Input>
-  class Stream {
-  public:
-    Stream(int id, std::vector<std::string> const& modules) : id_(id) {
-      token_ = reg_.produces<Input>();
-      int index = 1;
-      for (auto const& m : modules) {
-        reg_.beginModuleConstruction(index++);
-        workers_.push_back(edm::PluginFactory::create(m, reg_));
-      }
-    }
-
-    void run(std::vector<Input> const& events, edm::EventSetup const& es, EventCounter& counter,
-             std::atomic<int>& processed) {
-      for (int k; (k = counter.take()) >= 0;) {
-        edm::Event ev(id_, k + 1, reg_);
-        ev.emplace(token_, events[static_cast<size_t>(k) % events.size()]);
-        for (auto& w : workers_) w->doWork(ev, es);
-        processed.fetch_add(1);
-      }
-    }
-
-    void endJob() {
-      for (auto& w : workers_) w->doEndJob();
-    }
-
-  private:
-    int id_;
-    edm::ProductRegistry reg_;
-    edm::EDPutTokenT<Input> token_;
-    std::vector<std::unique_ptr<edm::Worker>> workers_;
-  };
-
+  // ------------------------------------------------------------ running the streams
+  // numberOfStreams EDM streams on a pool of numberOfThreads threads (src/serial/bin/main.cc:188-197,
+  // EventProcessor.cc:26-45); the first exception stops every stream and is rethrown here.
   template <typename Input>
   int process(std::vector<Input> const& events, std::vector<std::string> const& modules, edm::EventSetup const& es,
-              int numberOfStreams, int maxEvents, int runForMinutes, double& seconds) {
-    std::vector<std::unique_ptr<Stream<Input>>> streams;
-    for (int s = 0; s < numberOfStreams; ++s) streams.push_back(std::make_unique<Stream<Input>>(s, modules));
+              int numberOfThreads, int numberOfStreams, int maxEvents, int runForMinutes, double& seconds) {
+    std::vector<std::unique_ptr<edm::StreamSchedule<Input>>> streams;
+    for (int s = 0; s < numberOfStreams; ++s)
+      streams.push_back(std::make_unique<edm::StreamSchedule<Input>>(s, modules));
     EventCounter counter(maxEvents, runForMinutes, static_cast<int>(events.size()));
     std::atomic<int> processed{0};
     std::exception_ptr err;
     std::mutex m;
+    std::condition_variable cv;
+    int running = numberOfStreams;
     const auto t0 = std::chrono::steady_clock::now();
-    std::vector<std::thread> threads;
-    for (auto& s : streams)
-      threads.emplace_back([&, sp = s.get()] {
-        try {
-          sp->run(events, es, counter, processed);
-        } catch (...) {
+    {
+      edm::TaskPool pool(numberOfThreads);
+      for (auto& s : streams)
+        s->start(pool, events, es, [&counter] { return counter.take(); }, processed, [&](std::exception_ptr e) {
           std::lock_guard<std::mutex> g(m);
-          if (!err) err = std::current_exception();
-          counter.stop();  // stop the other streams
-        }
-      });
-    for (auto& t : threads) t.join();
+          if (e) {
+            if (!err) err = e;
+            counter.stop();  // the other streams stop at their next event
+          }
+          if (--running == 0) cv.notify_one();
+        });
+      std::unique_lock<std::mutex> lk(m);
+      cv.wait(lk, [&] { return running == 0; });
+    }
     seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (err) std::rethrow_exception(err);
     for (auto& s : streams) s->endJob();
@@ -235,7 +212,7 @@ namespace {
         << "       [--inputFile PATH] [--configFile PATH] [--validation] [--strictValidation] [--outputDir DIR]\n"
         << "       [--dumpDir DIR] [--plugin LIB] [--empty]\n\n"
         << "  --dim               2 (CLUE, default) or 3 (CLUE3D)\n"
-        << "  --numberOfThreads   threads (default 1); each runs one EDM stream\n"
+        << "  --numberOfThreads   threads running the EDM streams' modules (default 1; 0: one per core)\n"
         << "  --numberOfStreams   concurrent events (default: number of threads)\n"
         << "  --maxEvents         events to process (default: all events of the input file; 10 for a\n"
         << "                      toyDetector file)\n"
@@ -383,7 +360,7 @@ int main(int argc, char** argv) {
       if (events.empty()) throw std::runtime_error("input file holds no events");
       if (maxEvents < 0 && runForMinutes < 0) maxEvents = toy ? 10 : static_cast<int>(events.size());
       announce(maxEvents);
-      processed = process(events, modules, es, numberOfStreams, maxEvents, runForMinutes, seconds);
+      processed = process(events, modules, es, numberOfThreads, numberOfStreams, maxEvents, runForMinutes, seconds);
     } else {
       if (!configFile.empty()) {
         edm::ESPluginFactory::create("CLUEHIPTracksterizerESProducer", configFile)->produce(es);
@@ -403,7 +380,7 @@ int main(int argc, char** argv) {
       if (events.empty()) throw std::runtime_error("input file holds no events");
       if (maxEvents < 0 && runForMinutes < 0) maxEvents = static_cast<int>(events.size());
       announce(maxEvents);
-      processed = process(events, modules, es, numberOfStreams, maxEvents, runForMinutes, seconds);
+      processed = process(events, modules, es, numberOfThreads, numberOfStreams, maxEvents, runForMinutes, seconds);
     }
     const double cpu = cpu_seconds() - cpu0;
     std::cout << "Processed " << processed << " events in " << std::scientific << seconds << " seconds, throughput "

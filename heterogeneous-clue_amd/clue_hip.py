@@ -23,7 +23,8 @@ FUSED_CAPACITY = 10112
 # Every symbol include/clue_hip.h declares.
 EXPORTS = [
     "clue3d_default_params", "clue_create", "clue_destroy", "clue_last_error", "clue_device_count",
-    "clue_bytes_in_use", "clue_stream_create", "clue_stream_destroy", "clue_stream_sync", "clue_malloc_device",
+    "clue_bytes_in_use", "clue_stream_create", "clue_stream_destroy", "clue_stream_sync", "clue_stream_callback",
+    "clue_malloc_device", "clue_cache_alloc", "clue_cache_free", "clue_cache_release", "clue_cache_stats",
     "clue_free_device", "clue_malloc_pinned", "clue_free_pinned", "clue_memcpy_htod", "clue_memcpy_dtoh",
     "clue_memcpy_dtod",
     "clue_check", "clue_mark", "clue_elapsed_ms", "clue2d_run_device", "clue2d_run_host",
@@ -78,6 +79,11 @@ def load_library(path: Path = LIB_PATH) -> C.CDLL:
         "clue_stream_create": (VP, [VP]),
         "clue_stream_destroy": (I32, [VP, VP]),
         "clue_stream_sync": (I32, [VP, VP]),
+        "clue_stream_callback": (I32, [VP, VP, VP, VP]),
+        "clue_cache_alloc": (VP, [VP, C.c_uint64, VP]),
+        "clue_cache_free": (I32, [VP, VP]),
+        "clue_cache_release": (I32, [VP]),
+        "clue_cache_stats": (I32, [VP, VP, VP, VP, VP]),
         "clue_malloc_device": (VP, [VP, C.c_uint64]),
         "clue_free_device": (I32, [VP, VP]),
         "clue_malloc_pinned": (VP, [VP, C.c_uint64]),

@@ -413,6 +413,7 @@ void clue_destroy(clue_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipDeviceSynchronize();
+  cache_destroy(c);
   for (auto& kv : c->allocs) (void)hipFree(kv.first);
   c->allocs.clear();
   for (int k = 0; k < kMarks; ++k)
@@ -457,6 +458,13 @@ int32_t clue_stream_destroy(clue_ctx* c, void* s) {
 int32_t clue_stream_sync(clue_ctx* c, void* s) {
   if (!c) return CLUE_ERR_ARG;
   HIP_TRY(c, hipStreamSynchronize(pick(c, s)));
+  return CLUE_OK;
+}
+
+int32_t clue_stream_callback(clue_ctx* c, void* s, void (*fn)(void* user), void* user) {
+  if (!c) return CLUE_ERR_ARG;
+  if (!fn) return set_err(c, CLUE_ERR_ARG, "null callback");
+  HIP_TRY(c, hipLaunchHostFunc(pick(c, s), fn, user));
   return CLUE_OK;
 }
 

@@ -13,6 +13,7 @@ namespace clue {
 namespace detail {
 
 constexpr int kMarks = 16;
+struct DeviceCache;
 
 struct Workspace {
   void* base = nullptr;
@@ -36,6 +37,7 @@ struct clue_ctx {
   int cus = 256;                                          // compute units of the device
   hipStream_t scratchStream = nullptr;                    // stream of the last run_* that used the scratch
   std::unordered_map<void*, size_t> allocs;
+  clue::detail::DeviceCache* cache = nullptr;  // clue_cache_* blocks (created on first use)
 };
 
 namespace clue {
@@ -48,6 +50,8 @@ int dev_alloc(clue_ctx* c, void** p, size_t bytes);
 int dev_free(clue_ctx* c, void* p);
 // Grow w to at least `bytes` (synchronises the device before freeing the old buffer).
 int ensure(clue_ctx* c, Workspace& w, size_t bytes);
+// Frees the caching allocator's bookkeeping (its blocks are among the context's allocations).
+void cache_destroy(clue_ctx* c);
 
 }  // namespace detail
 }  // namespace clue

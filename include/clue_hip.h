@@ -91,12 +91,31 @@ uint64_t clue_bytes_in_use(const clue_ctx* ctx);
 void* clue_stream_create(clue_ctx* ctx);
 int32_t clue_stream_destroy(clue_ctx* ctx, void* stream);
 int32_t clue_stream_sync(clue_ctx* ctx, void* stream);
+/* Calls fn(user) once, on a runtime thread, after all work enqueued on `stream` before this call
+ * has finished, without blocking the caller (hipLaunchHostFunc). Replaces the CUDA backend's
+ * completion callback that hands an external-work module's holder back to the framework
+ * (src/cuda/CUDACore/ScopedContext.cc:94-98, cudaStreamAddCallback). fn must not call this library
+ * or HIP; it runs whether or not the enqueued kernels flagged an error (read those with clue_check). */
+int32_t clue_stream_callback(clue_ctx* ctx, void* stream, void (*fn)(void* user), void* user);
 
 /* Device / pinned-host memory owned by the caller, counted against the budget (device only). */
 void* clue_malloc_device(clue_ctx* ctx, uint64_t bytes);
 int32_t clue_free_device(clue_ctx* ctx, void* ptr);
 void* clue_malloc_pinned(clue_ctx* ctx, uint64_t bytes);
 int32_t clue_free_pinned(clue_ctx* ctx, void* ptr);
+/* Stream-ordered caching allocator over the context's device memory (replaces the reference's
+ * CachingDeviceAllocator, src/cuda/CUDACore/getCachingDeviceAllocator.h:14-94, used through
+ * cms::cuda::make_device_unique). Sizes round up to power-of-two bins of 2^8 .. 2^34 bytes (larger
+ * requests get exact blocks that are not cached). clue_cache_free gives a block back to the cache
+ * with an event recorded on the stream it was allocated on; a later clue_cache_alloc of the same bin
+ * reuses it at once on that stream, on another stream once the event has completed. Blocks count
+ * against the HBM budget; idle blocks are released when a new block would not fit, when idle bytes
+ * pass the limit (the budget, else 80 % of the free device memory at first use), by
+ * clue_cache_release, and with the context. */
+void* clue_cache_alloc(clue_ctx* ctx, uint64_t bytes, void* stream);
+int32_t clue_cache_free(clue_ctx* ctx, void* ptr);
+int32_t clue_cache_release(clue_ctx* ctx);
+int32_t clue_cache_stats(clue_ctx* ctx, uint64_t* live_bytes, uint64_t* idle_bytes, int64_t* hits, int64_t* misses);
 /* Asynchronous on `stream` (stream-ordered); pinned host memory makes them truly asynchronous. */
 int32_t clue_memcpy_htod(clue_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream);
 int32_t clue_memcpy_dtoh(clue_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream);

@@ -18,7 +18,7 @@ def test_framework_selftest():
     r = subprocess.run([str(PKG / "clue_fw_selftest")], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "framework self-test passed" in r.stdout
-    assert r.stdout.count("ok ") == 10
+    assert r.stdout.count("ok ") == 16
 
 
 def test_driver_help_and_bad_arguments():

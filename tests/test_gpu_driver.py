@@ -152,3 +152,36 @@ def test_driver_validation_failure_and_3d(tmp_path, oracle_lib):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "Validation not available for CLUE 3D" in r.stderr
+
+
+def test_driver_more_streams_than_threads(tmp_path, oracle_lib):
+    """4 EDM streams on 1 thread: the external-work modules hand the thread to the other streams while
+    an event is on the device; every event's outputs stay bit-exact."""
+    evs = [synth.clue2d_event(n, 90 + i) for i, n in enumerate((40000, 2000, 150000, 7000, 0, 60000))]
+    f = tmp_path / "raw2D.bin"
+    synth.write_raw(f, evs, dim=2)
+    cfg = tmp_path / "cfg.csv"
+    cfg.write_text("3,10,2,0\n")
+    print("case driver 4 streams 1 thread 2d", flush=True)
+    r = subprocess.run([str(PKG / "clue_hip_main"), "--inputFile", str(f), "--configFile", str(cfg),
+                        "--numberOfThreads", "1", "--numberOfStreams", "4", "--maxEvents", "12",
+                        "--dumpDir", str(tmp_path / "dump")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Processed 12 events" in r.stdout
+    for eid in range(1, 13):
+        ev = evs[(eid - 1) % len(evs)]
+        got = read_dump(tmp_path / "dump" / f"Event_{eid}.bin", 2)
+        want = oracle_lib.clue2d(ev["x"], ev["y"], ev["layer"], ev["weight"], 3.0, 10.0, 2.0)
+        helpers.assert_bit_exact(got, want, helpers.OUT2D, f"4-stream event {eid}")
+    evs3 = [synth.clue3d_event(20 + i) for i in range(3)]
+    f3 = tmp_path / "raw3D.bin"
+    synth.write_raw(f3, evs3, dim=3)
+    print("case driver 3 streams 1 thread 3d", flush=True)
+    r = subprocess.run([str(PKG / "clue_hip_main"), "--dim", "3", "--inputFile", str(f3), "--numberOfThreads", "1",
+                        "--numberOfStreams", "3", "--maxEvents", "6", "--dumpDir", str(tmp_path / "dump3")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    for eid in range(1, 7):
+        got = read_dump(tmp_path / "dump3" / f"Event_{eid}.bin", 3)
+        want = oracle_lib.clue3d(evs3[(eid - 1) % 3], oracle.clue3d_params())
+        helpers.assert_bit_exact(got, want, helpers.OUT3D, f"3-stream 3d event {eid}")
