@@ -68,3 +68,25 @@ def test_driver_without_gpu_fails_cleanly(tmp_path):
     r = run(["--dim", "2", "--inputFile", str(f), "--configFile", str(cfg)])
     assert r.returncode != 0
     assert "Caught std::runtime_error" in r.stdout and "no HIP device" in r.stdout
+
+
+def test_driver_event_counts_without_modules(tmp_path):
+    """--empty runs the framework alone (no GPU needed): maxEvents, runForMinutes (whole passes over the
+    file) and the toyDetector defaults (one event, 10 by default, 100 layers)."""
+    f = tmp_path / "raw2D.bin"
+    synth.write_raw(f, [synth.clue2d_event(300, i) for i in range(3)], dim=2)
+    cfg = tmp_path / "cfg.csv"
+    cfg.write_text("3,10,2,0\n")
+    r = run(["--inputFile", str(f), "--configFile", str(cfg), "--empty", "--maxEvents", "7",
+             "--numberOfThreads", "2", "--numberOfStreams", "5"])
+    assert r.returncode == 0 and "Processing 7 events, of which 5 concurrently, with 2 threads." in r.stdout
+    assert "Processed 7 events in" in r.stdout
+    r = run(["--inputFile", str(f), "--configFile", str(cfg), "--empty", "--runForMinutes", "0"])
+    assert r.returncode == 0 and "Processing for about 0 minutes with 1 concurrent events and 1 threads." in r.stdout
+    assert "Processed 3 events in" in r.stdout
+    toy = tmp_path / "data" / "input" / "toyDetector_2k.bin"
+    toy.parent.mkdir(parents=True)
+    synth.write_toy_detector(toy, synth.toy_detector_layer(50))
+    r = run(["--inputFile", str(toy), "--validation", "--empty"])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Processed 10 events in" in r.stdout and "dc = 20" in r.stderr
