@@ -68,9 +68,11 @@ using DigitL = Digit<k2dBinsPerLayer, kRadix>;
 
 // The density and nearest-higher kernels give each block ITEMS * kThreads consecutive sorted points
 // of one event (ITEMS points per thread). In x-major tile order everything their search boxes can
-// reach is one contiguous range of the sorted array: from the first bin of the lowest box to the end
-// of the highest box. When that range has at most STAGE points it is staged into LDS once and every
-// candidate is read from there; otherwise candidates are read from global memory. Either way the
+// reach lies in one contiguous range of the sorted array, whole columns from the lowest box's first
+// column to the highest box's last (block_reach, computed from the block's first and last keys, so
+// staging starts without waiting for the points' own bin-table lookups). When that range has at
+// most STAGE points it is staged into LDS once and every candidate is read from there; otherwise
+// candidates are read from global memory. Either way the
 // candidates are visited in the serial order, so the results do not depend on ITEMS, STAGE or which
 // path ran; the plan autotuner picks (ITEMS, STAGE) per kernel among kVariants2D by timing.
 template <int ITEMS>
@@ -113,10 +115,24 @@ __device__ __forceinline__ void make_runs(const int* __restrict__ T, uint32_t ke
   }
 }
 
-// First and one-past-last sorted positions the box reaches.
-__device__ __forceinline__ int runs_lo(const Runs& r) { return r.s[0]; }
-__device__ __forceinline__ int runs_hi(const int* __restrict__ T, const Runs& r) {
-  return r.n <= kMaxCols ? r.t[kMaxCols - 1] : T[r.L * k2dBinsPerLayer + r.xhi * k2dRows + r.yhi + 1];
+// The sorted range every search box (distance d) of the block's points [b0, b1) can reach, from the
+// block's first and last keys alone. A point in column xb has xbin(x - d) >= xb - m and
+// xbin(x + d) <= xb + m with m = ceil(d * rX + 0.01): the fp32 bin arithmetic errs by far less than
+// 0.01 of a bin, and clamped or NaN coordinates only narrow the box. The sorted array holds whole
+// columns in key order, so [first position of column max(0, xb0 - m) of layer L0, first position
+// after column min(99, xb1 + m) of layer L1) holds every candidate of every point of the block.
+__device__ __forceinline__ void block_reach(const int* __restrict__ T, const uint32_t* __restrict__ keyS, int b0,
+                                            int b1, float d, int& lo, int& hi) {
+  const uint32_t k0 = keyS[b0], k1 = keyS[b1 - 1];
+  const float rX = (float)k2dCols / (250.f - (-250.f));
+  const float md = d * rX + 0.01f;
+  const int m = md < 98.f ? (md > 0.f ? (int)ceilf(md) : 0) : k2dCols - 1;  // NaN: whole layers
+  const int L0 = (int)(k0 / (uint32_t)k2dBinsPerLayer), x0 = (int)(k0 / (uint32_t)k2dRows) % k2dCols;
+  const int L1 = (int)(k1 / (uint32_t)k2dBinsPerLayer), x1 = (int)(k1 / (uint32_t)k2dRows) % k2dCols;
+  const int c0 = x0 - m > 0 ? x0 - m : 0;
+  const int c1 = x1 + m < k2dCols - 1 ? x1 + m : k2dCols - 1;
+  lo = T[L0 * k2dBinsPerLayer + c0 * k2dRows];
+  hi = T[L1 * k2dBinsPerLayer + (c1 + 1) * k2dRows];
 }
 
 template <bool STAGED>
@@ -264,30 +280,25 @@ __global__ __launch_bounds__(kThreads) void k_density2d(const BlockDesc* __restr
                                                         float* __restrict__ rhoOut) {
   using Shape = CompShape<ITEMS>;
   __shared__ float4 s_st[STAGE];
-  __shared__ int s_red[2 * kWaves];
   const BlockDesc bd = blocks[blockIdx.x / Shape::kPerSort];
   const int b0 = bd.begin + (int)(blockIdx.x % Shape::kPerSort) * Shape::kHits;
   const int b1 = (b0 + Shape::kHits) < bd.end ? (b0 + Shape::kHits) : bd.end;
   if (b0 >= b1) return;  // uniform over the block
   const int* T = table + (int64_t)bd.event * stride;
+  int rlo, rhi;
+  block_reach(T, keyS, b0, b1, dc, rlo, rhi);
+  const bool staged = rhi - rlo <= STAGE;
+  if (staged) stage_range<STAGE>(xyw, s_st, rlo, rhi);
   Runs runs[ITEMS];
   float4 me[ITEMS];
-  int lo = 0x7fffffff, hi = -1;
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
     const int p = b0 + r * kThreads + (int)threadIdx.x;
     if (p < b1) {
       me[r] = xyw[p];
       make_runs(T, keyS[p], me[r].x, me[r].y, dc, runs[r]);
-      const int l = runs_lo(runs[r]), h = runs_hi(T, runs[r]);
-      lo = l < lo ? l : lo;
-      hi = h > hi ? h : hi;
     }
   }
-  int rlo, rhi;
-  block_range(lo, hi, s_red, rlo, rhi);
-  const bool staged = rhi - rlo <= STAGE;
-  if (staged) stage_range<STAGE>(xyw, s_st, rlo, rhi);
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
@@ -437,7 +448,6 @@ __global__ __launch_bounds__(kThreads) void k_nh2d(const BlockDesc* __restrict__
                                                    int* __restrict__ parent) {
   using Shape = CompShape<ITEMS>;
   __shared__ float4 s_st[STAGE];
-  __shared__ int s_red[2 * kWaves];
   const BlockDesc bd = blocks[blockIdx.x / Shape::kPerSort];
   const int b0 = bd.begin + (int)(blockIdx.x % Shape::kPerSort) * Shape::kHits;
   const int b1 = (b0 + Shape::kHits) < bd.end ? (b0 + Shape::kHits) : bd.end;
@@ -446,24 +456,20 @@ __global__ __launch_bounds__(kThreads) void k_nh2d(const BlockDesc* __restrict__
   const int evBase = evOff[bd.event];
   const float dm = odf * dc;
   const NhSrc src{xyw, rhoS};
+  int rlo, rhi;
+  block_reach(T, keyS, b0, b1, dm, rlo, rhi);
+  const bool staged = rhi - rlo <= STAGE;
+  if (staged) stage_range_nh<STAGE>(src, s_st, rlo, rhi);
   Runs runs[ITEMS];
   float4 me[ITEMS];
-  int lo = 0x7fffffff, hi = -1;
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
     const int p = b0 + r * kThreads + (int)threadIdx.x;
     if (p < b1) {
       me[r] = src.at(p);
       make_runs(T, keyS[p], me[r].x, me[r].y, dm, runs[r]);
-      const int l = runs_lo(runs[r]), h = runs_hi(T, runs[r]);
-      lo = l < lo ? l : lo;
-      hi = h > hi ? h : hi;
     }
   }
-  int rlo, rhi;
-  block_range(lo, hi, s_red, rlo, rhi);
-  const bool staged = rhi - rlo <= STAGE;
-  if (staged) stage_range_nh<STAGE>(src, s_st, rlo, rhi);
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
