@@ -36,6 +36,8 @@ struct Clue2DBuffers {
   int* i2 = nullptr;
   float4* xyw = nullptr;   // tile order: {x, y, weight, point index as int bits}
   float* rhoS = nullptr;   // tile order: rho
+  int4* reachD = nullptr;  // per 256-point tile: staging / table reach of the density boxes
+  int4* reachN = nullptr;  // the same for the nearest-higher boxes
   int* hist = nullptr;
   int* part = nullptr;
   int* table = nullptr;

@@ -106,11 +106,16 @@ void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DPara
       EmitLast2D{b.i2, b.k1, io.x, io.y, io.weight, b.xyw});
   const uint32_t* keyS = b.k1;
   build_table(keyS, b.blocks, b.nBlocks, b.evOff, b.E, k2dKeys, b.tableStride, b.table, b.tileStart, s);
-  launch_density2d(b.densityVariant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.table, b.tableStride, prm.dc,
-                   prm.dc2, b.rhoS, io.rho);
+  const int nTiles = b.nBlocks * kTilesPerSort;
+  k_reach2d<<<(nTiles + kThreads - 1) / kThreads, kThreads, 0, s>>>(b.blocks, nTiles, keyS, b.table, b.tableStride,
+                                                                    prm.dc, prm.outlierDeltaFactor, b.reachD,
+                                                                    b.reachN);
+  launch_density2d(b.densityVariant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.table, b.tableStride,
+                   (const int4*)b.reachD, prm.dc, prm.dc2, b.rhoS, io.rho);
   int* parent = reinterpret_cast<int*>(b.k2);
-  launch_nh2d(b.nhVariant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.rhoS, b.table, b.tableStride, b.evOff, prm.dc,
-              prm.rhoc, prm.outlierDeltaFactor, prm.dm2, io.delta, io.nearestHigher, io.isSeed, parent);
+  launch_nh2d(b.nhVariant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.rhoS, b.table, b.tableStride,
+              (const int4*)b.reachN, b.evOff, prm.dc, prm.rhoc, prm.outlierDeltaFactor, prm.dm2, io.delta,
+              io.nearestHigher, io.isSeed, parent);
   launch_labels(b.blocks, b.nBlocks, parent, io.isSeed, b.N, b.evOff, io.clusterIndex, b.seeds, b.part, s);
 }
 
@@ -120,11 +125,12 @@ void clue2d_enqueue_step(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2
   const uint32_t* keyS = b.k1;
   int* parent = reinterpret_cast<int*>(b.k2);
   if (which == 0)
-    launch_density2d(variant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.table, b.tableStride, prm.dc, prm.dc2,
-                     b.rhoS, io.rho);
+    launch_density2d(variant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.table, b.tableStride, (const int4*)b.reachD,
+                     prm.dc, prm.dc2, b.rhoS, io.rho);
   else
-    launch_nh2d(variant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.rhoS, b.table, b.tableStride, b.evOff, prm.dc,
-                prm.rhoc, prm.outlierDeltaFactor, prm.dm2, io.delta, io.nearestHigher, io.isSeed, parent);
+    launch_nh2d(variant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.rhoS, b.table, b.tableStride,
+                (const int4*)b.reachN, b.evOff, prm.dc, prm.rhoc, prm.outlierDeltaFactor, prm.dm2, io.delta,
+                io.nearestHigher, io.isSeed, parent);
 }
 
 void clue2d_variant_shape(int variant, int& items, int& stage) {
@@ -146,6 +152,8 @@ void clue2d_carve(void* base, int64_t N, int E, int nBlocks, Clue2DBuffers& b) {
   b.i2 = cv.take<int>(n);
   b.xyw = cv.take<float4>(n);
   b.rhoS = cv.take<float>(n);
+  b.reachD = cv.take<int4>((size_t)(nBlocks > 0 ? nBlocks : 1) * kTilesPerSort);
+  b.reachN = cv.take<int4>((size_t)(nBlocks > 0 ? nBlocks : 1) * kTilesPerSort);
   b.hist = cv.take<int>((size_t)kRadix * (nBlocks > 0 ? nBlocks : 1));
   const int64_t maxScan = (int64_t)kRadix * nBlocks > N ? (int64_t)kRadix * nBlocks : N;
   b.part = cv.take<int>((size_t)scan_partials(maxScan > 0 ? maxScan : 1));
@@ -164,6 +172,7 @@ size_t clue2d_workspace_bytes(int64_t N, int nEvents, int nBlocks) {
   for (int k = 0; k < 4; ++k) add(n * 4);                   // k1 i1 k2 i2
   add(n * 16);                                              // xyw
   add(n * 4);                                               // rhoS
+  for (int k = 0; k < 2; ++k) add((size_t)(nBlocks > 0 ? nBlocks : 1) * kTilesPerSort * sizeof(int4));  // reach
   add((size_t)kRadix * (size_t)(nBlocks > 0 ? nBlocks : 1) * 4);  // hist
   const int64_t maxScan = (int64_t)kRadix * nBlocks > N ? (int64_t)kRadix * nBlocks : N;
   add((size_t)scan_partials(maxScan > 0 ? maxScan : 1) * 4);  // partials

@@ -98,20 +98,30 @@ struct Runs {
   int n, L, xlo, xhi, ylo, yhi;
 };
 
-__device__ __forceinline__ void make_runs(const int* __restrict__ T, uint32_t key, float xi, float yi, float d,
-                                          Runs& r) {
+// The event's bin table, read from the block's LDS copy of entries [base, base + n) when the block
+// staged them, else from global memory (uniform over the block).
+struct TableView {
+  const int* g;
+  const int* s;
+  int base;
+  bool staged;
+  __device__ __forceinline__ int operator[](int b) const { return staged ? s[b - base] : g[b]; }
+};
+
+template <class Tab>
+__device__ __forceinline__ void make_runs(const Tab& T, uint32_t key, float xi, float yi, float d, Runs& r) {
   r.L = (int)(key / (uint32_t)k2dBinsPerLayer);
   r.xlo = xbin2d(xi - d);
   r.xhi = xbin2d(xi + d);
   r.ylo = ybin2d(yi - d);
   r.yhi = ybin2d(yi + d);
   r.n = r.xhi - r.xlo + 1;
-  const int* TL = T + r.L * k2dBinsPerLayer;
+  const int L0 = r.L * k2dBinsPerLayer;
 #pragma unroll
   for (int k = 0; k < kMaxCols; ++k) {
     const int xb = r.xlo + k < r.xhi ? r.xlo + k : r.xhi;
-    r.s[k] = TL[xb * k2dRows + r.ylo];
-    r.t[k] = TL[xb * k2dRows + r.yhi + 1];
+    r.s[k] = T[L0 + xb * k2dRows + r.ylo];
+    r.t[k] = T[L0 + xb * k2dRows + r.yhi + 1];
   }
 }
 
@@ -121,8 +131,10 @@ __device__ __forceinline__ void make_runs(const int* __restrict__ T, uint32_t ke
 // 0.01 of a bin, and clamped or NaN coordinates only narrow the box. The sorted array holds whole
 // columns in key order, so [first position of column max(0, xb0 - m) of layer L0, first position
 // after column min(99, xb1 + m) of layer L1) holds every candidate of every point of the block.
+// It also gives the bin-table entries [tlo, tlo + tn) that the boxes' lookups can touch (the
+// largest lookup is the entry after the last reachable column's last row).
 __device__ __forceinline__ void block_reach(const int* __restrict__ T, const uint32_t* __restrict__ keyS, int b0,
-                                            int b1, float d, int& lo, int& hi) {
+                                            int b1, float d, int& lo, int& hi, int& tlo, int& tn) {
   const uint32_t k0 = keyS[b0], k1 = keyS[b1 - 1];
   const float rX = (float)k2dCols / (250.f - (-250.f));
   const float md = d * rX + 0.01f;
@@ -131,8 +143,67 @@ __device__ __forceinline__ void block_reach(const int* __restrict__ T, const uin
   const int L1 = (int)(k1 / (uint32_t)k2dBinsPerLayer), x1 = (int)(k1 / (uint32_t)k2dRows) % k2dCols;
   const int c0 = x0 - m > 0 ? x0 - m : 0;
   const int c1 = x1 + m < k2dCols - 1 ? x1 + m : k2dCols - 1;
-  lo = T[L0 * k2dBinsPerLayer + c0 * k2dRows];
-  hi = T[L1 * k2dBinsPerLayer + (c1 + 1) * k2dRows];
+  tlo = L0 * k2dBinsPerLayer + c0 * k2dRows;
+  const int thi = L1 * k2dBinsPerLayer + (c1 + 1) * k2dRows;
+  tn = thi - tlo + 1;
+  lo = T[tlo];
+  hi = T[thi];
+}
+
+// block_reach of every 256-point tile of the sorted batch for the density (dc) and nearest-higher
+// (dm) distances, one thread per tile, as {first sorted position, end position, first table entry,
+// last table entry}. A compute block of several consecutive tiles of one event reaches from its
+// first tile's starts to its last tile's ends (both grow with the sorted keys), so the neighbour
+// kernels read two records instead of walking the keys and the table before they can stage.
+constexpr int kTilesPerSort = kSortBlock / kThreads;
+__global__ __launch_bounds__(kThreads) void k_reach2d(const BlockDesc* __restrict__ blocks, int nTiles,
+                                                      const uint32_t* __restrict__ keyS,
+                                                      const int* __restrict__ table, int64_t stride, float dc,
+                                                      float odf, int4* __restrict__ reachD,
+                                                      int4* __restrict__ reachN) {
+  const int tile = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (tile >= nTiles) return;
+  const BlockDesc bd = blocks[tile / kTilesPerSort];
+  const int b0 = bd.begin + (tile % kTilesPerSort) * kThreads;
+  if (b0 >= bd.end) return;  // past the event's last point: no compute block reads it
+  const int b1 = b0 + kThreads < bd.end ? b0 + kThreads : bd.end;
+  const int* T = table + (int64_t)bd.event * stride;
+  int lo, hi, tlo, tn;
+  block_reach(T, keyS, b0, b1, dc, lo, hi, tlo, tn);
+  reachD[tile] = make_int4(lo, hi, tlo, tlo + tn - 1);
+  block_reach(T, keyS, b0, b1, odf * dc, lo, hi, tlo, tn);
+  reachN[tile] = make_int4(lo, hi, tlo, tlo + tn - 1);
+}
+
+// The reach of compute block `blk` of ITEMS tiles covering sorted points [b0, b1).
+template <int ITEMS>
+__device__ __forceinline__ void compute_reach(const int4* __restrict__ reach, int blk, int b0, int b1, int& lo,
+                                              int& hi, int& tlo, int& tn) {
+  constexpr int kPerSort = kSortBlock / (kThreads * ITEMS);
+  const int t0 = (blk / kPerSort) * kTilesPerSort + (blk % kPerSort) * ITEMS;
+  const int t1 = t0 + (b1 - b0 - 1) / kThreads;
+  const int4 r0 = reach[t0], r1 = reach[t1];
+  lo = r0.x;
+  hi = r1.y;
+  tlo = r0.z;
+  tn = r1.w - r0.z + 1;
+}
+
+// Entries [tlo, tlo + tn) of the table into LDS (tn <= kTabStage), all loads of a thread in flight.
+constexpr int kTabStage = 768;
+__device__ __forceinline__ void stage_table(const int* __restrict__ T, int* st, int tlo, int tn) {
+  constexpr int kPer = kTabStage / kThreads;
+  int v[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int k = u * kThreads + (int)threadIdx.x;
+    v[u] = T[tlo + (k < tn ? k : tn - 1)];
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int k = u * kThreads + (int)threadIdx.x;
+    if (k < tn) st[k] = v[u];
+  }
 }
 
 template <bool STAGED>
@@ -275,31 +346,35 @@ template <int ITEMS, int STAGE>
 __global__ __launch_bounds__(kThreads) void k_density2d(const BlockDesc* __restrict__ blocks,
                                                         const uint32_t* __restrict__ keyS,
                                                         const float4* __restrict__ xyw,
-                                                        const int* __restrict__ table, int64_t stride, float dc,
-                                                        float dc2, float* __restrict__ rhoS,
-                                                        float* __restrict__ rhoOut) {
+                                                        const int* __restrict__ table, int64_t stride,
+                                                        const int4* __restrict__ reach, float dc, float dc2,
+                                                        float* __restrict__ rhoS, float* __restrict__ rhoOut) {
   using Shape = CompShape<ITEMS>;
   __shared__ float4 s_st[STAGE];
+  __shared__ int s_tab[kTabStage];
   const BlockDesc bd = blocks[blockIdx.x / Shape::kPerSort];
   const int b0 = bd.begin + (int)(blockIdx.x % Shape::kPerSort) * Shape::kHits;
   const int b1 = (b0 + Shape::kHits) < bd.end ? (b0 + Shape::kHits) : bd.end;
   if (b0 >= b1) return;  // uniform over the block
   const int* T = table + (int64_t)bd.event * stride;
-  int rlo, rhi;
-  block_reach(T, keyS, b0, b1, dc, rlo, rhi);
+  int rlo, rhi, tlo, tn;
+  compute_reach<ITEMS>(reach, (int)blockIdx.x, b0, b1, rlo, rhi, tlo, tn);
   const bool staged = rhi - rlo <= STAGE;
   if (staged) stage_range<STAGE>(xyw, s_st, rlo, rhi);
-  Runs runs[ITEMS];
+  const TableView tab{T, s_tab, tlo, tn <= kTabStage};
+  if (tab.staged) stage_table(T, s_tab, tlo, tn);
   float4 me[ITEMS];
+  uint32_t key[ITEMS];
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
     const int p = b0 + r * kThreads + (int)threadIdx.x;
-    if (p < b1) {
-      me[r] = xyw[p];
-      make_runs(T, keyS[p], me[r].x, me[r].y, dc, runs[r]);
-    }
+    me[r] = xyw[p < b1 ? p : b0];
+    key[r] = keyS[p < b1 ? p : b0];
   }
   __syncthreads();
+  Runs runs[ITEMS];
+#pragma unroll
+  for (int r = 0; r < ITEMS; ++r) make_runs(tab, key[r], me[r].x, me[r].y, dc, runs[r]);
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
     const int p = b0 + r * kThreads + (int)threadIdx.x;
@@ -441,13 +516,15 @@ template <int ITEMS, int STAGE>
 __global__ __launch_bounds__(kThreads) void k_nh2d(const BlockDesc* __restrict__ blocks,
                                                    const uint32_t* __restrict__ keyS,
                                                    const float4* __restrict__ xyw, const float* __restrict__ rhoS,
-                                                   const int* __restrict__ table,
-                                                   int64_t stride, const int* __restrict__ evOff, float dc, float rhoc,
+                                                   const int* __restrict__ table, int64_t stride,
+                                                   const int4* __restrict__ reach, const int* __restrict__ evOff,
+                                                   float dc, float rhoc,
                                                    float odf, float dm2, float* __restrict__ deltaOut,
                                                    int32_t* __restrict__ nhOut, int32_t* __restrict__ isSeedOut,
                                                    int* __restrict__ parent) {
   using Shape = CompShape<ITEMS>;
   __shared__ float4 s_st[STAGE];
+  __shared__ int s_tab[kTabStage];
   const BlockDesc bd = blocks[blockIdx.x / Shape::kPerSort];
   const int b0 = bd.begin + (int)(blockIdx.x % Shape::kPerSort) * Shape::kHits;
   const int b1 = (b0 + Shape::kHits) < bd.end ? (b0 + Shape::kHits) : bd.end;
@@ -456,21 +533,24 @@ __global__ __launch_bounds__(kThreads) void k_nh2d(const BlockDesc* __restrict__
   const int evBase = evOff[bd.event];
   const float dm = odf * dc;
   const NhSrc src{xyw, rhoS};
-  int rlo, rhi;
-  block_reach(T, keyS, b0, b1, dm, rlo, rhi);
+  int rlo, rhi, tlo, tn;
+  compute_reach<ITEMS>(reach, (int)blockIdx.x, b0, b1, rlo, rhi, tlo, tn);
   const bool staged = rhi - rlo <= STAGE;
   if (staged) stage_range_nh<STAGE>(src, s_st, rlo, rhi);
-  Runs runs[ITEMS];
+  const TableView tab{T, s_tab, tlo, tn <= kTabStage};
+  if (tab.staged) stage_table(T, s_tab, tlo, tn);
   float4 me[ITEMS];
+  uint32_t key[ITEMS];
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
     const int p = b0 + r * kThreads + (int)threadIdx.x;
-    if (p < b1) {
-      me[r] = src.at(p);
-      make_runs(T, keyS[p], me[r].x, me[r].y, dm, runs[r]);
-    }
+    me[r] = src.at(p < b1 ? p : b0);
+    key[r] = keyS[p < b1 ? p : b0];
   }
   __syncthreads();
+  Runs runs[ITEMS];
+#pragma unroll
+  for (int r = 0; r < ITEMS; ++r) make_runs(tab, key[r], me[r].x, me[r].y, dm, runs[r]);
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
     const int p = b0 + r * kThreads + (int)threadIdx.x;

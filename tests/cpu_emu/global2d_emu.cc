@@ -75,18 +75,23 @@ int main(int argc, char** argv) {
           EmitLast2D{i2.data(), k1.data(), in.x.data(), in.y.data(), in.w.data(), xyw.data()});
     });
     build_table(k1.data(), blocks.data(), nb, evOff.data(), E, k2dKeys, stride, table.data(), tileStart.data());
+    const int nTiles = nb * kTilesPerSort;
+    std::vector<int4> reachD(nTiles, int4{P, P, P, P}), reachN(nTiles, int4{P, P, P, P});
+    emu::launch((nTiles + kThreads - 1) / kThreads, kThreads, [&] {
+      k_reach2d(blocks.data(), nTiles, k1.data(), table.data(), stride, dc, odf, reachD.data(), reachN.data());
+    });
     const int dv = argc > 3 ? atoi(argv[3]) : 0, nv = argc > 4 ? atoi(argv[4]) : 0;
     dispatch(dv, [&](auto I, auto S) {
       emu::launch(CompShape<I>::kPerSort * nb, kThreads, [&] {
-        k_density2d<I, S>(blocks.data(), k1.data(), xyw.data(), table.data(), stride, dc, dc2, rhoS.data(),
-                          rho.data());
+        k_density2d<I, S>(blocks.data(), k1.data(), xyw.data(), table.data(), stride, reachD.data(), dc, dc2,
+                          rhoS.data(), rho.data());
       });
     });
     int* parent = reinterpret_cast<int*>(k2.data());
     dispatch(nv, [&](auto I, auto S) {
       emu::launch(CompShape<I>::kPerSort * nb, kThreads, [&] {
-        k_nh2d<I, S>(blocks.data(), k1.data(), xyw.data(), rhoS.data(), table.data(), stride, evOff.data(), dc, rhoc,
-                     odf, dm2, delta.data(), nh.data(), isSeed.data(), parent);
+        k_nh2d<I, S>(blocks.data(), k1.data(), xyw.data(), rhoS.data(), table.data(), stride, reachN.data(),
+                     evOff.data(), dc, rhoc, odf, dm2, delta.data(), nh.data(), isSeed.data(), parent);
       });
     });
     SeedRankHost seeds(N);
