@@ -107,9 +107,8 @@ void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DPara
   const uint32_t* keyS = b.k1;
   build_table(keyS, b.blocks, b.nBlocks, b.evOff, b.E, k2dKeys, b.tableStride, b.table, b.tileStart, s);
   const int nTiles = b.nBlocks * kTilesPerSort;
-  k_reach2d<<<(nTiles + kThreads - 1) / kThreads, kThreads, 0, s>>>(b.blocks, nTiles, keyS, b.table, b.tableStride,
-                                                                    prm.dc, prm.outlierDeltaFactor, b.reachD,
-                                                                    b.reachN);
+  k_reach2d<<<(nTiles + kReachThreads - 1) / kReachThreads, kReachThreads, 0, s>>>(
+      b.blocks, nTiles, keyS, b.table, b.tableStride, prm.dc, prm.outlierDeltaFactor, b.reachD, b.reachN);
   launch_density2d(b.densityVariant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.table, b.tableStride,
                    (const int4*)b.reachD, prm.dc, prm.dc2, b.rhoS, io.rho);
   int* parent = reinterpret_cast<int*>(b.k2);

@@ -69,7 +69,7 @@ using DigitL = Digit<k2dBinsPerLayer, kRadix>;
 // The density and nearest-higher kernels give each block ITEMS * kThreads consecutive sorted points
 // of one event (ITEMS points per thread). In x-major tile order everything their search boxes can
 // reach lies in one contiguous range of the sorted array, whole columns from the lowest box's first
-// column to the highest box's last (block_reach, computed from the block's first and last keys, so
+// column to the highest box's last (reach_columns, computed from the block's first and last keys, so
 // staging starts without waiting for the points' own bin-table lookups). When that range has at
 // most STAGE points it is staged into LDS once and every candidate is read from there; otherwise
 // candidates are read from global memory. Either way the
@@ -130,12 +130,10 @@ __device__ __forceinline__ void make_runs(const Tab& T, uint32_t key, float xi, 
 // xbin(x + d) <= xb + m with m = ceil(d * rX + 0.01): the fp32 bin arithmetic errs by far less than
 // 0.01 of a bin, and clamped or NaN coordinates only narrow the box. The sorted array holds whole
 // columns in key order, so [first position of column max(0, xb0 - m) of layer L0, first position
-// after column min(99, xb1 + m) of layer L1) holds every candidate of every point of the block.
-// It also gives the bin-table entries [tlo, tlo + tn) that the boxes' lookups can touch (the
-// largest lookup is the entry after the last reachable column's last row).
-__device__ __forceinline__ void block_reach(const int* __restrict__ T, const uint32_t* __restrict__ keyS, int b0,
-                                            int b1, float d, int& lo, int& hi, int& tlo, int& tn) {
-  const uint32_t k0 = keyS[b0], k1 = keyS[b1 - 1];
+// after column min(99, xb1 + m) of layer L1) holds every candidate of every point of the block, and
+// the bin-table entries [tlo, thi] hold every lookup of their boxes (the largest is the entry after
+// the last reachable column's last row). reach_columns gives tlo and thi; the range is T[tlo], T[thi].
+__device__ __forceinline__ void reach_columns(uint32_t k0, uint32_t k1, float d, int& tlo, int& thi) {
   const float rX = (float)k2dCols / (250.f - (-250.f));
   const float md = d * rX + 0.01f;
   const int m = md < 98.f ? (md > 0.f ? (int)ceilf(md) : 0) : k2dCols - 1;  // NaN: whole layers
@@ -144,23 +142,21 @@ __device__ __forceinline__ void block_reach(const int* __restrict__ T, const uin
   const int c0 = x0 - m > 0 ? x0 - m : 0;
   const int c1 = x1 + m < k2dCols - 1 ? x1 + m : k2dCols - 1;
   tlo = L0 * k2dBinsPerLayer + c0 * k2dRows;
-  const int thi = L1 * k2dBinsPerLayer + (c1 + 1) * k2dRows;
-  tn = thi - tlo + 1;
-  lo = T[tlo];
-  hi = T[thi];
+  thi = L1 * k2dBinsPerLayer + (c1 + 1) * k2dRows;
 }
 
-// block_reach of every 256-point tile of the sorted batch for the density (dc) and nearest-higher
+// The reach of every 256-point tile of the sorted batch for the density (dc) and nearest-higher
 // (dm) distances, one thread per tile, as {first sorted position, end position, first table entry,
 // last table entry}. A compute block of several consecutive tiles of one event reaches from its
 // first tile's starts to its last tile's ends (both grow with the sorted keys), so the neighbour
 // kernels read two records instead of walking the keys and the table before they can stage.
 constexpr int kTilesPerSort = kSortBlock / kThreads;
-__global__ __launch_bounds__(kThreads) void k_reach2d(const BlockDesc* __restrict__ blocks, int nTiles,
-                                                      const uint32_t* __restrict__ keyS,
-                                                      const int* __restrict__ table, int64_t stride, float dc,
-                                                      float odf, int4* __restrict__ reachD,
-                                                      int4* __restrict__ reachN) {
+constexpr int kReachThreads = 64;  // one wave per block: the few hundred blocks spread over all CUs
+__global__ __launch_bounds__(kReachThreads) void k_reach2d(const BlockDesc* __restrict__ blocks, int nTiles,
+                                                           const uint32_t* __restrict__ keyS,
+                                                           const int* __restrict__ table, int64_t stride, float dc,
+                                                           float odf, int4* __restrict__ reachD,
+                                                           int4* __restrict__ reachN) {
   const int tile = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (tile >= nTiles) return;
   const BlockDesc bd = blocks[tile / kTilesPerSort];
@@ -168,11 +164,13 @@ __global__ __launch_bounds__(kThreads) void k_reach2d(const BlockDesc* __restric
   if (b0 >= bd.end) return;  // past the event's last point: no compute block reads it
   const int b1 = b0 + kThreads < bd.end ? b0 + kThreads : bd.end;
   const int* T = table + (int64_t)bd.event * stride;
-  int lo, hi, tlo, tn;
-  block_reach(T, keyS, b0, b1, dc, lo, hi, tlo, tn);
-  reachD[tile] = make_int4(lo, hi, tlo, tlo + tn - 1);
-  block_reach(T, keyS, b0, b1, odf * dc, lo, hi, tlo, tn);
-  reachN[tile] = make_int4(lo, hi, tlo, tlo + tn - 1);
+  const uint32_t k0 = keyS[b0], k1 = keyS[b1 - 1];
+  int tloD, thiD, tloN, thiN;
+  reach_columns(k0, k1, dc, tloD, thiD);
+  reach_columns(k0, k1, odf * dc, tloN, thiN);
+  const int loD = T[tloD], hiD = T[thiD], loN = T[tloN], hiN = T[thiN];
+  reachD[tile] = make_int4(loD, hiD, tloD, thiD);
+  reachN[tile] = make_int4(loN, hiN, tloN, thiN);
 }
 
 // The reach of compute block `blk` of ITEMS tiles covering sorted points [b0, b1).

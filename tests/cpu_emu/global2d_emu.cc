@@ -77,7 +77,7 @@ int main(int argc, char** argv) {
     build_table(k1.data(), blocks.data(), nb, evOff.data(), E, k2dKeys, stride, table.data(), tileStart.data());
     const int nTiles = nb * kTilesPerSort;
     std::vector<int4> reachD(nTiles, int4{P, P, P, P}), reachN(nTiles, int4{P, P, P, P});
-    emu::launch((nTiles + kThreads - 1) / kThreads, kThreads, [&] {
+    emu::launch((nTiles + kReachThreads - 1) / kReachThreads, kReachThreads, [&] {
       k_reach2d(blocks.data(), nTiles, k1.data(), table.data(), stride, dc, odf, reachD.data(), reachN.data());
     });
     const int dv = argc > 3 ? atoi(argv[3]) : 0, nv = argc > 4 ? atoi(argv[4]) : 0;
