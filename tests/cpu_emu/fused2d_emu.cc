@@ -18,6 +18,10 @@ using namespace clue::fused;
 namespace {
 
 float sqrt_le(float d) {  // largest v with sqrtf(v) <= d
+  // as the library's sqrt_le_threshold: NaN passes nothing, a negative d nothing either
+  if (d != d) return d;
+  if (d < 0.f) return -1.f;
+  if (d == INFINITY) return INFINITY;
   uint32_t lo = 0, hi = 0x7f800000u;
   while (lo < hi) {
     const uint32_t mid = lo + (hi - lo + 1) / 2;

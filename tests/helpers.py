@@ -108,3 +108,31 @@ def clue2d_csv_lines(ev: dict, out: dict) -> list[str]:
                               _g(out["rho"][i]), _g(999 if d > 999 else d), str(int(out["nearestHigher"][i])),
                               str(int(out["isSeed"][i])), str(int(out["clusterIndex"][i]))]))
     return rows
+
+
+EXTREME_PARAMS = [(3.0, 10.0, 2.0), (0.0, 1.0, 2.0), (-1.0, 1.0, 2.0), (300.0, 10.0, 2.0), (3.0, 10.0, 1e6),
+                  (float("nan"), 10.0, 2.0)]
+
+
+def extreme_event(seed: int = 5, n_normal: int = 500) -> dict:
+    """A CLUE2D event whose points include NaN, infinite and huge coordinates, points just inside and
+    outside the [-250, 250) grid and on bin edges, mixed with ordinary blob points."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    c = rng.uniform(-240, 240, (12, 2))
+    pts = c[rng.integers(0, 12, n_normal)] + rng.normal(0, 1.5, (n_normal, 2))
+    special = [np.nan, np.inf, -np.inf, 1e30, -1e30, 1e9, -1e9, 250.0, -250.0, 249.99998, -250.00002, 1.07374e10,
+               0.0, -0.0, 5.0, -5.0, 245.0, 1e-40]
+    sx, sy = [], []
+    for a in special:
+        for b in (0.0, 3.0, np.nan, 250.0, -1e30):
+            sx.append(a)
+            sy.append(b)
+            sx.append(b)
+            sy.append(a)
+    x = np.concatenate([pts[:, 0], sx]).astype(np.float32)
+    y = np.concatenate([pts[:, 1], sy]).astype(np.float32)
+    n = x.size
+    layer = rng.integers(0, 4, n).astype(np.int32) * 25
+    weight = rng.exponential(1.0, n).astype(np.float32)
+    p = rng.permutation(n)
+    return {"x": x[p], "y": y[p], "layer": layer[p], "weight": weight[p]}

@@ -127,6 +127,18 @@ def test_emulated_global_variants(emulator, tmp_path, variants):
     helpers.assert_bit_exact(got, want, helpers.OUT2D, f"emulated variants {variants}")
 
 
+@pytest.mark.parametrize("params", helpers.EXTREME_PARAMS)
+def test_emulated_global_extremes(emulator, tmp_path, oracle_lib, params):
+    """NaN, infinite and out-of-grid coordinates with zero, negative, NaN and very large distances:
+    the staging reach and the table columns staged per block must still hold every lookup."""
+    ev = helpers.extreme_event()
+    dc, rhoc, odf = params
+    err, _, got = run_emulated(EXE_GLOBAL, tmp_path, ev, np.array([0, ev["x"].size]), dc, rhoc, odf)
+    assert err == 0
+    want = oracle_lib.clue2d(ev["x"], ev["y"], ev["layer"], ev["weight"], dc, rhoc, odf)
+    helpers.assert_bit_exact(got, want, helpers.OUT2D, f"emulated extremes {params}")
+
+
 # ------------------------------------------------------------------ CLUE3D
 
 FIELDS_3D_F = ("z", "eta", "phi", "r_over_absz", "radius", "energy")

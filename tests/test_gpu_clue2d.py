@@ -133,3 +133,27 @@ def test_c3_batch_properties_full_size(dev, oracle_lib):
         want = oracle_lib.clue2d(packed["x"][a:b], packed["y"][a:b], packed["layer"][a:b], packed["weight"][a:b])
         helpers.assert_bit_exact({k: v[a:b] for k, v in out.items()}, want, helpers.OUT2D, f"c3 event {e}")
     plan.destroy()
+
+
+@pytest.mark.parametrize("params", helpers.EXTREME_PARAMS)
+def test_extreme_coordinates_and_distances(dev, oracle_lib, params):
+    """NaN, infinite and out-of-grid coordinates, zero / negative / NaN / very large distances, alone and
+    batched beside an ordinary 50000-hit event (staged and unstaged neighbourhoods in one launch)."""
+    dc, rhoc, odf = params
+    p = clue_hip.params2d(dc, rhoc, odf)
+    ev = helpers.extreme_event()
+    print(f"case clue2d extremes {params} n={ev['x'].size}", flush=True)
+    got = dev.clue2d_host(ev, p)
+    want = oracle_lib.clue2d(ev["x"], ev["y"], ev["layer"], ev["weight"], dc, rhoc, odf)
+    helpers.assert_bit_exact(got, want, helpers.OUT2D, f"extremes {params}")
+    packed, offs = synth.pack([ev, synth.clue2d_event(50000, 3), helpers.extreme_event(seed=6)])
+    N = int(offs[-1])
+    din = dev.upload_soa(packed, clue_hip.FIELDS_2D_IN)
+    dout = dev.alloc_soa(N, clue_hip.FIELDS_2D_OUT)
+    dev.clue2d_batch(din, dout, offs, p)
+    dev.check()
+    got = dev.download_soa(dout)
+    for a in list(din.values()) + list(dout.values()):
+        a.free()
+    helpers.assert_bit_exact(got, oracle_lib.clue2d_batch(packed, offs, dc, rhoc, odf), helpers.OUT2D,
+                             f"extremes batched {params}")
