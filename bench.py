@@ -108,7 +108,17 @@ def cpu_baseline(packed, offs, p, gpu_out):
     return {"value": round(n / t / 1e6, 3), "unit": "Mhits/s", "cores": threads, "kind": kind,
             "sample": f"all {offs.size - 1} events of the workload ({n} hits), one event per thread at a time",
             "seconds": round(t, 3), "labels_bit_exact_vs_gpu": bool(same),
-            "single_thread_one_event_mhits_per_s": single}
+            "single_thread_one_event_mhits_per_s": single, "nproc": os.cpu_count(), "cpu_model": cpu_model()}
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def per_event_run3d(dev, din, dout, offs, p):
