@@ -127,11 +127,12 @@ def test_emulated_global_variants(emulator, tmp_path, variants):
     helpers.assert_bit_exact(got, want, helpers.OUT2D, f"emulated variants {variants}")
 
 
-@pytest.mark.parametrize("params", helpers.EXTREME_PARAMS)
+@pytest.mark.parametrize("params", [helpers.EXTREME_PARAMS[k] for k in (4, 5)])
 def test_emulated_global_extremes(emulator, tmp_path, oracle_lib, params):
-    """NaN, infinite and out-of-grid coordinates with zero, negative, NaN and very large distances:
-    the staging reach and the table columns staged per block must still hold every lookup."""
-    ev = helpers.extreme_event()
+    """NaN, infinite and out-of-grid coordinates with a NaN and a very large distance:
+    the staging reach and the table columns staged per block must still hold every lookup (the GPU
+    test runs every parameter set)."""
+    ev = helpers.extreme_event(n_normal=200)
     dc, rhoc, odf = params
     err, _, got = run_emulated(EXE_GLOBAL, tmp_path, ev, np.array([0, ev["x"].size]), dc, rhoc, odf)
     assert err == 0
