@@ -18,10 +18,10 @@ namespace edm {
   }
 
   void TaskPool::push(std::function<void()> task) {
-    {
-      std::lock_guard<std::mutex> g(m_);
-      queue_.push_back(std::move(task));
-    }
+    // Notify under the lock: a thread outside the pool (a device callback) may push the task that
+    // lets the owner finish and destroy the pool, which cannot happen while this lock is held.
+    std::lock_guard<std::mutex> g(m_);
+    queue_.push_back(std::move(task));
     cv_.notify_one();
   }
 
