@@ -1,5 +1,4 @@
-// Conditions products by type: put<T> (throws on a duplicate), get<T> (throws if missing) and
-// find<T> (nullptr if missing).
+// Conditions products by type: put<T> (throws on a duplicate) and get<T> (throws if missing).
 // API of the reference's edm::EventSetup (src/serial/Framework/EventSetup.h:29-53).
 #ifndef CLUE_FW_EventSetup_h
 #define CLUE_FW_EventSetup_h
@@ -25,13 +24,6 @@ namespace edm {
       if (it == products_.end())
         throw std::runtime_error(std::string("Product of type ") + typeid(T).name() + " is not produced");
       return *static_cast<Holder<T> const&>(*it->second).obj;
-    }
-
-    // The product of type T, or nullptr when none was put (for optional conditions).
-    template <typename T>
-    T const* find() const {
-      auto it = products_.find(std::type_index(typeid(T)));
-      return it == products_.end() ? nullptr : static_cast<Holder<T> const&>(*it->second).obj.get();
     }
 
   private:

@@ -43,7 +43,12 @@ private:
       if (len != n) throw std::runtime_error("CLUEHIPTracksterizer: ClusterCollection columns differ in length");
     if (n == 0) return;
     Parameters3D par;
-    if (auto const* p3 = eventSetup.find<Parameters3D>()) par = *p3;  // else the reference's defaults
+    try {
+      par = eventSetup.get<Parameters3D>();
+    } catch (std::runtime_error const&) {
+      // no CLUE3D conditions module configured: the reference's defaults (EventSetup::get is the
+      // reference framework's only accessor, so the module builds there unchanged)
+    }
     clue3d_params p;
     p.densitySiblingLayers = par.densitySiblingLayers;
     p.densityXYDistanceSqr = par.densityXYDistanceSqr;

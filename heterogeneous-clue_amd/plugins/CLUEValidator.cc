@@ -146,7 +146,12 @@ private:
     auto const& out = eventSetup.get<OutputDirPath>();
     auto const& pc = event.get(resultsToken_);
     Parameters const& par = eventSetup.get<Parameters>();
-    ValidationMode const* mode = eventSetup.find<ValidationMode>();
+    bool strict = false;
+    try {
+      strict = eventSetup.get<ValidationMode>().strict;
+    } catch (std::runtime_error const&) {
+      // optional: the reference's driver puts no ValidationMode
+    }
     const std::string refName = reference_file_for(out.outFile.string());
     std::ostringstream log;  // one write per event, so concurrent streams do not interleave lines
     if (refName.empty()) {
@@ -174,7 +179,7 @@ private:
         require_equal("nearestHigher", pc.nearestHigher, ref.pc.nearestHigher, log);
         require_equal("isSeed", pc.isSeed, ref.pc.isSeed, log);
         require_same_clusters(pc, ref.pc, log);
-        if (mode && mode->strict) require_same_rows(pc, ref, log);
+        if (strict) require_same_rows(pc, ref, log);
         log << "CLUE output is correct!\n";
       } else {
         log << "Parameters different from the ones used in the reference file, cannot check results\n";
