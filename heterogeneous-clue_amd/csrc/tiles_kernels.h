@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kThreads) void k_table(const uint32_t* __restrict__
   }
   __syncthreads();
   int* out = table + (int64_t)e * stride + b0;
-  for (int j = threadIdx.x; j < b1 - b0; j += kThreads) out[j] = s_first[tpad(j)];
+  for (int j = threadIdx.x; j < b1 - b0; j += kThreads) __builtin_nontemporal_store(s_first[tpad(j)], &out[j]);
 }
 
 
