@@ -26,6 +26,11 @@ std::vector<BlockDesc> make_blocks(const int64_t* offsets, int nEvents, int bloc
 void scan_exclusive(const int* in, int* out, int64_t n, int* part, hipStream_t s) {
   if (n <= 0) return;
   const int np = scan_partials(n);
+  if (np <= kScanTile) {  // one or two launches: the down pass sums the partials it needs itself
+    if (np > 1) k_scan_reduce<<<np, kThreads, 0, s>>>(in, n, part);
+    k_scan_down_direct<<<np, kThreads, 0, s>>>(in, out, n, part);
+    return;
+  }
   k_scan_reduce<<<np, kThreads, 0, s>>>(in, n, part);
   k_scan_partials<<<1, kThreads, 0, s>>>(part, np);
   k_scan_down<<<np, kThreads, 0, s>>>(in, out, n, part);

@@ -46,9 +46,8 @@ __global__ __launch_bounds__(kThreads) void k_scan_partials(int* __restrict__ pa
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_scan_down(const int* in, int* out, int64_t n,
-                                                        const int* __restrict__ part) {
-  __shared__ int s_tmp[kWaves];
+// Exclusive scan of the kScanTile items of this block, starting from `offset`.
+__device__ __forceinline__ void scan_tile(const int* in, int* out, int64_t n, int offset, int* s_tmp) {
   const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
   int v[kScanItems];
   int sum = 0;
@@ -58,12 +57,36 @@ __global__ __launch_bounds__(kThreads) void k_scan_down(const int* in, int* out,
     sum += v[k];
   }
   int total;
-  int run = part[blockIdx.x] + block_exclusive_sum(sum, s_tmp, total);
+  int run = offset + block_exclusive_sum(sum, s_tmp, total);
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
     if (base + k < n) out[base + k] = run;
     run += v[k];
   }
+}
+
+// `part` holds the exclusive scan of the block totals (k_scan_partials).
+__global__ __launch_bounds__(kThreads) void k_scan_down(const int* in, int* out, int64_t n,
+                                                        const int* __restrict__ part) {
+  __shared__ int s_tmp[kWaves];
+  scan_tile(in, out, n, part[blockIdx.x], s_tmp);
+}
+
+// `part` holds the block totals themselves (at most kScanTile of them): each block adds up those of
+// the blocks before it, one coalesced round of kScanItems loads per thread, so that no single-block
+// pass over the partials sits between the reduce and this kernel.
+__global__ __launch_bounds__(kThreads) void k_scan_down_direct(const int* in, int* out, int64_t n,
+                                                               const int* __restrict__ part) {
+  __shared__ int s_tmp[kWaves];
+  int pre = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int j = k * kThreads + (int)threadIdx.x;
+    pre += j < (int)blockIdx.x ? part[j] : 0;
+  }
+  int offset;
+  block_exclusive_sum(pre, s_tmp, offset);
+  scan_tile(in, out, n, offset, s_tmp);
 }
 
 

@@ -14,9 +14,15 @@ namespace emu_host {
 
 using namespace clue;
 
-inline void scan_exclusive(const int* in, int* out, int64_t n, int* part) {
+// scan_exclusive's launch sequence; the direct two-launch form up to `directMax` partials.
+inline void scan_exclusive(const int* in, int* out, int64_t n, int* part, int directMax = kScanTile) {
   if (n <= 0) return;
   const int np = scan_partials(n);
+  if (np <= directMax) {
+    if (np > 1) emu::launch(np, kThreads, [&] { k_scan_reduce(in, n, part); });
+    emu::launch(np, kThreads, [&] { k_scan_down_direct(in, out, n, part); });
+    return;
+  }
   emu::launch(np, kThreads, [&] { k_scan_reduce(in, n, part); });
   emu::launch(1, kThreads, [&] { k_scan_partials(part, np); });
   emu::launch(np, kThreads, [&] { k_scan_down(in, out, n, part); });

@@ -209,3 +209,26 @@ def test_emulated_clue3d_params(emulator, tmp_path, oracle_lib, kw):
     err, got = run_emulated3d(tmp_path, ev, np.array([0, ev["layer"].size]), p)
     assert err == 0
     helpers.assert_bit_exact(got, oracle_lib.clue3d(ev, p), helpers.OUT3D, f"emulated clue3d {kw}")
+
+
+EXE_SCAN = EMU / "_build" / "scan_emu"
+
+
+@pytest.mark.parametrize("n,direct_max", [(0, 2048), (1, 2048), (2047, 2048), (2049, 2048), (2048 * 12 + 3, 2048),
+                                          (2048 * 12 + 3, 0), (2048 * 6, 6), (2048 * 6 + 1, 6)])
+def test_emulated_scan(emulator, tmp_path, n, direct_max):
+    """The device-wide exclusive scan, in place and out of place, in its two-launch form (each block
+    sums the partials before it) and its three-launch form (a single-block scan of the partials), with
+    the partials buffer poisoned, against numpy's cumulative sum."""
+    v = np.random.Generator(np.random.PCG64(n)).integers(-3, 50, n).astype(np.int32)
+    src, dst = tmp_path / "scan_in.bin", tmp_path / "scan_out.bin"
+    with open(src, "wb") as f:
+        f.write(np.int64(n).tobytes())
+        f.write(v.tobytes())
+    r = subprocess.run([str(EXE_SCAN), str(src), str(dst), str(direct_max)], capture_output=True, text=True,
+                       timeout=600)
+    assert r.returncode == 0, r.stderr[-4000:]
+    d = np.fromfile(dst, np.int32)
+    want = (np.cumsum(v, dtype=np.int64) - v).astype(np.int32)
+    assert np.array_equal(d[:n], want)
+    assert np.array_equal(d[n:], want)

@@ -157,3 +157,29 @@ def test_extreme_coordinates_and_distances(dev, oracle_lib, params):
         a.free()
     helpers.assert_bit_exact(got, oracle_lib.clue2d_batch(packed, offs, dc, rhoc, odf), helpers.OUT2D,
                              f"extremes batched {params}")
+
+
+def test_batch_of_68_million_hits(dev, oracle_lib):
+    """68 copies of one 1e6-hit event: 4.26e6 radix histogram counters, more partial sums than the
+    two-launch scan takes, so the radix passes run the three-launch scan; every copy must equal the
+    oracle's result for the event alone."""
+    ev = synth.clue2d_event(1_000_000, 11)
+    want = oracle_lib.clue2d(ev["x"], ev["y"], ev["layer"], ev["weight"])
+    E, n = 68, ev["x"].size
+    packed = {k: np.tile(v, E) for k, v in ev.items()}
+    offs = np.arange(E + 1, dtype=np.int64) * n
+    print(f"case clue2d 68-copy batch n={E * n}", flush=True)
+    din = dev.upload_soa(packed, clue_hip.FIELDS_2D_IN)
+    del packed
+    dout = dev.alloc_soa(E * n, clue_hip.FIELDS_2D_OUT)
+    dev.clue2d_batch(din, dout, offs)
+    dev.check()
+    for k in helpers.OUT2D:
+        got = dout[k].download().reshape(E, n)
+        w = np.asarray(want[k]).reshape(1, n)
+        if got.dtype == np.float32:
+            got, w = got.view(np.int32), w.view(np.int32)
+        bad = np.nonzero((got != w).any(axis=1))[0]
+        assert bad.size == 0, f"{k}: copies {bad[:8]} differ from the oracle"
+    for a in list(din.values()) + list(dout.values()):
+        a.free()

@@ -8,6 +8,7 @@ oracle's output for a synthetic toyDetector layer, at the parameters those files
 (CLUEValidator.cc:42-199): 1e-3 absolute on floats, delta clamped at 999, seed-mapped cluster ids,
 results only checked at those parameters.
 """
+import os
 import subprocess
 from pathlib import Path
 
@@ -29,9 +30,13 @@ def harness():
     if not PLUGINS.exists():
         pytest.skip("plugin library not built")
     HARNESS.parent.mkdir(exist_ok=True)
+    # built under a name of this process and renamed into place, so that test processes running side
+    # by side never execute a half-written binary
+    tmp = HARNESS.with_name(f"{HARNESS.name}.{os.getpid()}")
     subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-I", str(PKG / "framework"), "-I", str(REPO / "include"),
-                    "-o", str(HARNESS), str(HARNESS_SRC), "-L", str(PKG), "-lclue_framework", "-ldl",
+                    "-o", str(tmp), str(HARNESS_SRC), "-L", str(PKG), "-lclue_framework", "-ldl",
                     f"-Wl,-rpath,{PKG}"], check=True, timeout=120)
+    os.replace(tmp, HARNESS)
     return HARNESS
 
 
