@@ -223,9 +223,12 @@ int plan_route_first(clue_plan* pl, hipStream_t s) {
   return CLUE_OK;
 }
 
-// CLUE2D plan autotuning: one full run (so this launch produces its results), then each variant of
-// the density and of the nearest-higher kernel timed in place (best of 3) on the same data; the
-// fastest of each is kept. Re-running either kernel rewrites its outputs with the same values.
+// CLUE2D / CLUE3D plan autotuning: one full run (so this launch produces its results), then each
+// variant of the density and of the nearest-higher kernel timed in place on the same data (best of
+// kTuneRounds interleaved rounds); the fastest of each is kept. Re-running either kernel rewrites its
+// outputs with the same values.
+constexpr int kTuneRounds = 5;
+static_assert(kClue2DVariants <= 8 && kClue3DVariants <= 8, "plan_autotune keeps at most 8 timings");
 int plan_autotune(clue_plan* pl, hipStream_t s) {
   clue_ctx* c = pl->ctx;
   const bool d3 = pl->dim == 3;
@@ -240,10 +243,11 @@ int plan_autotune(clue_plan* pl, hipStream_t s) {
   const int nVariants = d3 ? kClue3DVariants : kClue2DVariants;
   int best[2] = {0, 0};
   for (int which = 0; which < 2; ++which) {
-    float bestMs = 0.f;
-    for (int v = 0; v < nVariants; ++v) {
-      float ms = 0.f;
-      for (int rep = 0; rep < 4; ++rep) {  // the first run of each variant warms it up
+    float ms[8] = {};
+    // a warm-up round, then kTuneRounds rounds over all variants in turn (best of each), so that a
+    // slow stretch of the device weighs on every variant alike
+    for (int round = 0; round <= kTuneRounds; ++round) {
+      for (int v = 0; v < nVariants; ++v) {
         HIP_TRY(c, hipEventRecord(e0, s));
         if (d3)
           clue3d_enqueue_step(pl->b3, pl->io3, pl->p3, which, v, s);
@@ -253,13 +257,11 @@ int plan_autotune(clue_plan* pl, hipStream_t s) {
         HIP_TRY(c, hipEventSynchronize(e1));
         float t = 0.f;
         HIP_TRY(c, hipEventElapsedTime(&t, e0, e1));
-        if (rep == 1 || (rep > 1 && t < ms)) ms = t;
-      }
-      if (v == 0 || ms < bestMs) {
-        bestMs = ms;
-        best[which] = v;
+        if (round == 1 || (round > 1 && t < ms[v])) ms[v] = t;
       }
     }
+    for (int v = 1; v < nVariants; ++v)
+      if (ms[v] < ms[best[which]]) best[which] = v;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
