@@ -48,17 +48,14 @@ struct EmitLast2D {
   const float* y;
   const float* w;
   float4* xyw;
-  struct Payload {
-    int idx;
-    float x, y, w;
-  };
-  __device__ __forceinline__ Payload load(int i) const {
-    const int idx = iIn[i];
-    return Payload{idx, x[idx], y[idx], w[idx]};
-  }
-  __device__ __forceinline__ void store(int pos, uint32_t k, const Payload& v) const {
+  // the point index passes through LDS; x, y and w are gathered once it is in sorted order
+  using Payload = int;
+  using Out = float4;
+  __device__ __forceinline__ Payload load(int i) const { return iIn[i]; }
+  __device__ __forceinline__ Out fetch(int idx) const { return make_float4(x[idx], y[idx], w[idx], __int_as_float(idx)); }
+  __device__ __forceinline__ void store(int pos, uint32_t k, const Out& v) const {
     kOut[pos] = k;
-    xyw[pos] = make_float4(v.x, v.y, v.w, __int_as_float(v.idx));
+    xyw[pos] = v;
   }
 };
 

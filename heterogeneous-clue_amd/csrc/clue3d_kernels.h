@@ -58,18 +58,20 @@ struct EmitLast3D {
   const float* en;
   const float* z;
   float4* cre;
-  struct Payload {
+  // the cluster index passes through LDS; the four fields are gathered once it is in sorted order
+  using Payload = int;
+  struct Out {
     int idx;
-    float roz, phi, en, z;
+    float4 v;
   };
-  __device__ __forceinline__ Payload load(int i) const {
-    const int idx = iIn[i];
-    return Payload{idx, roz[idx], phi[idx], en[idx], z[idx]};
+  __device__ __forceinline__ Payload load(int i) const { return iIn[i]; }
+  __device__ __forceinline__ Out fetch(int idx) const {
+    return Out{idx, make_float4(roz[idx], phi[idx], en[idx], z[idx])};
   }
-  __device__ __forceinline__ void store(int pos, uint32_t k, const Payload& v) const {
+  __device__ __forceinline__ void store(int pos, uint32_t k, const Out& o) const {
     kOut[pos] = k;
-    iOut[pos] = v.idx;
-    cre[pos] = make_float4(v.roz, v.phi, v.en, v.z);
+    iOut[pos] = o.idx;
+    cre[pos] = o.v;
   }
 };
 

@@ -44,8 +44,10 @@ struct EmitPoint2D {
   const float* w;
   float4* lg;
   using Payload = float4;
+  using Out = float4;
   __device__ __forceinline__ Payload load(int i) const { return make_float4(x[i], y[i], w[i], __int_as_float(i)); }
-  __device__ __forceinline__ void store(int pos, uint32_t, const Payload& v) const { lg[pos] = v; }
+  __device__ __forceinline__ Out fetch(const Payload& v) const { return v; }
+  __device__ __forceinline__ void store(int pos, uint32_t, const Out& v) const { lg[pos] = v; }
 };
 
 using DigitLayer = Digit<1, kRadix>;

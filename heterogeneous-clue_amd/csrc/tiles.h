@@ -87,10 +87,12 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(const BlockDesc* __rest
 }
 
 // Emit::load(i) reads what item i carries (issued for all of a thread's items before ranking);
-// Emit::store(pos, key, payload) writes it at its sorted position. The block first ranks its items
-// stably by digit (wave ballot match + per-wave counts, round by round in input order), places
-// them in LDS in (digit, rank) order, and then writes each digit's run to its global position with
-// consecutive threads on consecutive positions.
+// Emit::fetch(payload) turns it into what is written (it may gather from global memory: only the
+// small payload then passes through LDS) and Emit::store(pos, key, out) writes that at the item's
+// sorted position. The block first ranks its items stably by digit (wave ballot match + per-wave
+// counts, round by round in input order), places them in LDS in (digit, rank) order, and then
+// writes each digit's run to its global position with consecutive threads on consecutive positions,
+// all of a thread's fetches issued before its stores.
 template <class Src, class D, class Emit>
 __global__ __launch_bounds__(kThreads) void k_radix_scatter(const BlockDesc* __restrict__ blocks, Src src,
                                                             const int* __restrict__ histScanned, Emit emit) {
@@ -162,11 +164,24 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const BlockDesc* __r
     }
   }
   __syncthreads();
-  for (int j = tid; j < nItems; j += kThreads) {
-    const uint32_t k = s_key[j];
-    const int d = D::get(k);
-    emit.store(s_base[d] + (j - s_start[d]), k, s_pay[j]);
+  typename Emit::Out out[kSortItems];
+  uint32_t okey[kSortItems];
+  int opos[kSortItems];
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int j = r * kThreads + tid;
+    okey[r] = 0u;
+    opos[r] = 0;
+    if (j < nItems) {
+      okey[r] = s_key[j];
+      const int d = D::get(okey[r]);
+      opos[r] = s_base[d] + (j - s_start[d]);
+      out[r] = emit.fetch(s_pay[j]);
+    }
   }
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r)
+    if (r * kThreads + tid < nItems) emit.store(opos[r], okey[r], out[r]);
 }
 
 // First pass: the item is its input position.
@@ -174,8 +189,10 @@ struct EmitFirst {
   uint32_t* kOut;
   int* iOut;
   using Payload = int;
+  using Out = int;
   __device__ __forceinline__ Payload load(int i) const { return i; }
-  __device__ __forceinline__ void store(int pos, uint32_t k, Payload v) const {
+  __device__ __forceinline__ Out fetch(Payload v) const { return v; }
+  __device__ __forceinline__ void store(int pos, uint32_t k, Out v) const {
     kOut[pos] = k;
     iOut[pos] = v;
   }
@@ -187,8 +204,10 @@ struct EmitMid {
   uint32_t* kOut;
   int* iOut;
   using Payload = int;
+  using Out = int;
   __device__ __forceinline__ Payload load(int i) const { return iIn[i]; }
-  __device__ __forceinline__ void store(int pos, uint32_t k, Payload v) const {
+  __device__ __forceinline__ Out fetch(Payload v) const { return v; }
+  __device__ __forceinline__ void store(int pos, uint32_t k, Out v) const {
     kOut[pos] = k;
     iOut[pos] = v;
   }
