@@ -89,8 +89,9 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(const BlockDesc* __rest
 // Emit::load(i) reads what item i carries (issued for all of a thread's items before ranking);
 // Emit::fetch(payload) turns it into what is written (it may gather from global memory: only the
 // small payload then passes through LDS) and Emit::store(pos, key, out) writes that at the item's
-// sorted position. The block first ranks its items stably by digit (wave ballot match + per-wave
-// counts, round by round in input order), places them in LDS in (digit, rank) order, and then
+// sorted position. The block first ranks its items stably by digit (each wave its own consecutive
+// items: wave ballot match and a wave-private count per digit, then one prefix over the waves),
+// places them in LDS in (digit, rank) order, and then
 // writes each digit's run to its global position with consecutive threads on consecutive positions,
 // all of a thread's fetches issued before its stores.
 template <class Src, class D, class Emit>
@@ -107,46 +108,55 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const BlockDesc* __r
   const BlockDesc bd = blocks[xcd_block((int)blockIdx.x, (int)gridDim.x)];
   const int tid = (int)threadIdx.x, lane = lane_id(), wave = tid / kWave;
   const int nItems = bd.end - bd.begin;
+  // wave w holds items [w * kWaveItems, (w + 1) * kWaveItems) of the block, round r its lanes'
+  // consecutive items r * kWave + lane: input order is (wave, round, lane)
+  constexpr int kWaveItems = kSortItems * kWave;
+  const int first = bd.begin + wave * kWaveItems + lane;
   uint32_t key[kSortItems];
   Payload pay[kSortItems];
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    const int i = bd.begin + r * kThreads + tid;
+    const int i = first + r * kWave;
     key[r] = i < bd.end ? src(i) : 0u;
   }
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    const int i = bd.begin + r * kThreads + tid;
+    const int i = first + r * kWave;
     pay[r] = emit.load(i < bd.end ? i : bd.begin);
   }
-  if (tid < kRadix) {
-    s_base[tid] = histScanned[bd.histBase + tid * bd.nbEvent + bd.bLocal];
-    s_run[tid] = 0;
-  }
+  if (tid < kRadix) s_base[tid] = histScanned[bd.histBase + tid * bd.nbEvent + bd.bLocal];
   for (int k = lane; k < kRadix; k += kWave) s_cnt[wave][k] = 0;
-  __syncthreads();
-  int rank[kSortItems];  // position among the block's items of the same digit
+  // Stable rank inside the wave: per round, the lanes of one digit (ballot match) take consecutive
+  // ranks after that digit's running count in the wave; the digit's lowest lane advances the count
+  // and broadcasts the old value (the shuffle orders every update of a round before the next
+  // round's reads). Invalid items carry digit kRadix - 1, which no key has, and are never placed.
+  int rank[kSortItems];
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    const bool valid = bd.begin + r * kThreads + tid < bd.end;
+    const bool valid = first + r * kWave < bd.end;
     const int d = valid ? D::get(key[r]) : kRadix - 1;
     const uint64_t m = match_digit7(d);
     const int before = __popcll(m & ((1ull << lane) - 1ull));
-    if (valid && before == 0) s_cnt[wave][d] = (int)__popcll(m);
-    __syncthreads();
-    if (tid < kRadix) {
-      int run = s_run[tid];
-#pragma unroll
-      for (int w = 0; w < kWaves; ++w) {
-        s_off[w][tid] = run;
-        run += s_cnt[w][tid];
-      }
-      s_run[tid] = run;
+    int old = 0;
+    if (before == 0) {
+      old = s_cnt[wave][d];
+      s_cnt[wave][d] = old + (int)__popcll(m);
     }
-    __syncthreads();
-    rank[r] = s_off[wave][d] + before;
-    for (int k = lane; k < kRadix; k += kWave) s_cnt[wave][k] = 0;
+    old = __shfl(old, __builtin_ctzll(m), kWave);  // m holds this lane, so it is not 0
+    rank[r] = old + before;
   }
+  __syncthreads();
+  // the waves' counts of each digit, in wave order: offsets and the block total
+  if (tid < kRadix) {
+    int run = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      s_off[w][tid] = run;
+      run += s_cnt[w][tid];
+    }
+    s_run[tid] = run;
+  }
+  __syncthreads();
   // start of each digit's run within the block (exclusive scan of the block's digit counts)
   if (tid < kWave) {
     int v0 = s_run[2 * tid], v1 = s_run[2 * tid + 1];
@@ -157,8 +167,9 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const BlockDesc* __r
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    if (bd.begin + r * kThreads + tid < bd.end) {
-      const int pos = s_start[D::get(key[r])] + rank[r];
+    if (first + r * kWave < bd.end) {
+      const int d = D::get(key[r]);
+      const int pos = s_start[d] + s_off[wave][d] + rank[r];
       s_key[pos] = key[r];
       s_pay[pos] = pay[r];
     }
