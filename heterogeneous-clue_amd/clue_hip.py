@@ -19,6 +19,7 @@ ERRORS = {-1: "CLUE_ERR_ARG", -2: "CLUE_ERR_HIP", -3: "CLUE_ERR_BUDGET", -4: "CL
 PATH_AUTO, PATH_FUSED, PATH_GLOBAL = 0, 1, 2
 PLAN_GRAPH, PLAN_AUTOTUNE = 1, 2
 FUSED_CAPACITY = 10112
+NOALLOC_MAX_POINTS = 1048576
 
 # Every symbol include/clue_hip.h declares.
 EXPORTS = [
@@ -32,7 +33,7 @@ EXPORTS = [
     "clue3d_run_batch_device", "clue3d_plan_create", "clue_plan_launch", "clue_plan_destroy",
     "clue_plan_workspace_bytes", "clue_selftest_arith", "clue_server_create", "clue_server_destroy",
     "clue_server_submit2d", "clue_server_submit3d", "clue_server_wait", "clue_server_stats",
-    "clue_server_last_error", "clue_set_path", "clue_plan_path", "clue_plan_tuning",
+    "clue_server_last_error", "clue_set_path", "clue_plan_path", "clue_plan_tuning", "clue2d_run_device_noalloc",
 ]
 
 
@@ -95,6 +96,7 @@ def load_library(path: Path = LIB_PATH) -> C.CDLL:
         "clue_mark": (I32, [VP, I32, VP]),
         "clue_elapsed_ms": (C.c_float, [VP, I32, I32]),
         "clue2d_run_device": (I32, [VP, C.POINTER(Clue2DParams), I32] + [VP] * 9 + [VP]),
+        "clue2d_run_device_noalloc": (I32, [VP, C.POINTER(Clue2DParams), I32] + [VP] * 9 + [VP]),
         "clue2d_run_host": (I32, [VP, C.POINTER(Clue2DParams), I32] + [VP] * 9),
         "clue2d_run_batch_device": (I32, [VP, C.POINTER(Clue2DParams), I32, VP] + [VP] * 9 + [VP]),
         "clue2d_plan_create": (VP, [VP, C.POINTER(Clue2DParams), I32, VP] + [VP] * 9 + [I32]),
@@ -246,6 +248,12 @@ class Device:
         p = p or params2d()
         self._chk(self.lib.clue2d_run_device(self.ctx, C.byref(p), n, *[din[k].ptr for k in FIELDS_2D_IN],
                                              *[dout[k].ptr for k in FIELDS_2D_OUT], stream))
+
+    def clue2d_device_noalloc(self, din: dict, dout: dict, n: int, p: Clue2DParams | None = None, stream=None):
+        """clue2d_run_device on the library's fixed scratch (no allocation; n <= NOALLOC_MAX_POINTS)."""
+        p = p or params2d()
+        self._chk(self.lib.clue2d_run_device_noalloc(self.ctx, C.byref(p), n, *[din[k].ptr for k in FIELDS_2D_IN],
+                                                     *[dout[k].ptr for k in FIELDS_2D_OUT], stream))
 
     def clue2d_batch(self, din: dict, dout: dict, offs: np.ndarray, p: Clue2DParams | None = None, stream=None):
         p = p or params2d()

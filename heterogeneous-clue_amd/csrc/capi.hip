@@ -4,6 +4,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <unordered_map>
@@ -110,6 +111,35 @@ bool check_offsets(const int64_t* offs, int n) {
   for (int e = 0; e < n; ++e)
     if (offs[e + 1] < offs[e]) return false;
   return offs[n] < (int64_t)0x7fffffff;
+}
+
+// ---------------------------------------------------------------- allocation-free CLUE2D scratch
+// clue2d_run_device_noalloc carves its workspace out of this fixed array instead of allocating: one
+// copy per GPU, held until the process ends. In host code `scratch` names a host-side stand-in, not
+// device memory, so it is never passed to a kernel or a copy; the device address of each GPU's copy
+// is looked up once per device (noalloc_setup, called from clue_create with the device current)
+// and only that address is used.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMu;
+void* g_scratchDev[kMaxDevices] = {};
+
+int noalloc_setup(clue_ctx* c) {
+  if (c->device < 0 || c->device >= kMaxDevices) return set_err(c, CLUE_ERR_ARG, "device index above 63");
+  std::lock_guard<std::mutex> g(g_scratchMu);
+  if (g_scratchDev[c->device]) return CLUE_OK;
+  void* p = nullptr;
+  HIP_TRY(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return set_err(c, CLUE_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+  g_scratchDev[c->device] = p;
+  return CLUE_OK;
+}
+
+void* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> g(g_scratchMu);
+  return g_scratchDev[device];
 }
 
 __global__ void k_single_event(int n, int nb, BlockDesc* blocks, int* evOff, int* evBlk) {
@@ -405,6 +435,12 @@ clue_ctx* clue_create(int32_t device, uint64_t hbm_budget_bytes) {
     return nullptr;
   }
   c->dErr = reinterpret_cast<int*>(p);
+  if (noalloc_setup(c) != CLUE_OK) {
+    (void)hipFree(p);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return nullptr;
+  }
   fill_i32(c->dErr, 0, 64, c->stream);
   for (int k = 0; k < kMarks; ++k) (void)hipEventCreate(&c->marks[k]);
   (void)hipStreamSynchronize(c->stream);
@@ -568,6 +604,39 @@ int32_t clue2d_run_device(clue_ctx* c, const clue2d_params* p, int32_t n, const 
   const int64_t offs[2] = {0, n};
   Clue2DIO io{x, y, layer, weight, rho, delta, nh, isSeed, clusterIndex};
   return run2d(c, to2d(p), 1, offs, io, s, true);
+}
+
+int32_t clue2d_run_device_noalloc(clue_ctx* c, const clue2d_params* p, int32_t n, const float* x, const float* y,
+                                  const int32_t* layer, const float* weight, float* rho, float* delta, int32_t* nh,
+                                  int32_t* isSeed, int32_t* clusterIndex, void* stream) {
+  if (!c || !p || n < 0) return set_err(c, CLUE_ERR_ARG, "bad argument");
+  if (n == 0) return CLUE_OK;
+  if (!x || !y || !layer || !weight || !rho || !delta || !nh || !isSeed || !clusterIndex)
+    return set_err(c, CLUE_ERR_ARG, "null buffer");
+  if (n > CLUE_NOALLOC_MAX_POINTS)
+    return set_err(c, CLUE_ERR_CAPACITY, "clue2d_run_device_noalloc: " + std::to_string(n) +
+                                             " points exceed its fixed scratch (at most " +
+                                             std::to_string(CLUE_NOALLOC_MAX_POINTS) + "); use clue2d_run_device");
+  void* base = noalloc_scratch(c->device);
+  if (!base) return set_err(c, CLUE_ERR_HIP, "the scratch array of this device has no device address");
+  const int nb = (n + kSortBlock - 1) / kSortBlock;
+  const size_t need = clue2d_workspace_bytes(n, 1, nb);
+  if (need > kNoAllocScratchBytes)
+    return set_err(c, CLUE_ERR_CAPACITY, "clue2d_run_device_noalloc: workspace of " + std::to_string(need) +
+                                             " bytes exceeds the fixed scratch");
+  hipStream_t s = pick(c, stream);
+  // Every workspace word the launch sequence reads is written earlier in the same sequence (block
+  // table, histograms, scans, bin table, reach records, seed words), so the previous call's contents
+  // need no clearing.
+  Clue2DBuffers b;
+  clue2d_carve(base, n, 1, nb, b);
+  b.err = c->dErr;
+  int rc = single_setup(c, n, nb, b.blocks, b.evOff, s);
+  if (rc) return rc;
+  Clue2DIO io{x, y, layer, weight, rho, delta, nh, isSeed, clusterIndex};
+  clue2d_enqueue(b, io, to2d(p), s);
+  HIP_TRY(c, hipGetLastError());
+  return CLUE_OK;
 }
 
 int32_t clue2d_run_batch_device(clue_ctx* c, const clue2d_params* p, int32_t E, const int64_t* offsets,

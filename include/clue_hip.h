@@ -33,7 +33,7 @@ extern "C" {
 #define CLUE_ERR_LAYER (-4)    /* an input layer outside the tile array (2D: [0,100), 3D: [0,94)) */
 #define CLUE_ERR_BIN (-5)      /* CLUE3D: a phi so far outside [-pi, pi] that its bin leaves the tile */
 #define CLUE_ERR_TIMEOUT (-6)  /* clue_server_wait: the request did not finish in time */
-#define CLUE_ERR_CAPACITY (-7) /* CLUE2D fused path: an (event, layer) with more points than it holds */
+#define CLUE_ERR_CAPACITY (-7) /* a fixed capacity exceeded: CLUE2D fused path layer, no-alloc scratch */
 
 /* CLUE2D execution paths (clue_set_path). Both give the serial backend's results bit for bit.
  *   CLUE_PATH_GLOBAL the batch is sorted into bins in device memory (no size limit);
@@ -146,6 +146,18 @@ int32_t clue_set_path(clue_ctx* ctx, int32_t dim, int32_t path);
 int32_t clue2d_run_device(clue_ctx* ctx, const clue2d_params* p, int32_t n, const float* x, const float* y,
                           const int32_t* layer, const float* weight, float* rho, float* delta, int32_t* nearestHigher,
                           int32_t* isSeed, int32_t* clusterIndex, void* stream);
+
+/* clue2d_run_device without any allocation: no reference counterpart. For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, results and stream semantics as clue2d_run_device; the workspace is a
+ * fixed 64 MiB device array of the library (one per GPU, held until the process ends) instead of
+ * the context's scratch, so events are limited to CLUE_NOALLOC_MAX_POINTS points (CLUE_ERR_CAPACITY
+ * above). Every call on a GPU shares that GPU's array: calls on one stream run one after another,
+ * but calls that overlap on one GPU (other streams or contexts) race. */
+#define CLUE_NOALLOC_MAX_POINTS 1048576
+int32_t clue2d_run_device_noalloc(clue_ctx* ctx, const clue2d_params* p, int32_t n, const float* x, const float* y,
+                                  const int32_t* layer, const float* weight, float* rho, float* delta,
+                                  int32_t* nearestHigher, int32_t* isSeed, int32_t* clusterIndex, void* stream);
 
 /* One event, host SoA in and out, synchronous: the body of an EDProducer::produce
  * (src/serial/plugin-CLUEClusterizer/CLUESerialClusterizer.cc:29-36). */
