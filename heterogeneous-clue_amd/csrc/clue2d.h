@@ -16,6 +16,7 @@ struct Clue2DParams {
   // because a correctly rounded sqrt is monotone; see sqrt_le_threshold().
   float dc2;
   float dm2;
+  float dm2up;  // the float after dm2 (the nearest-higher scan's initial running minimum)
 };
 
 // Largest float v >= 0 with sqrtf(v) <= d (sqrtf correctly rounded, hence non-decreasing), so that
@@ -35,7 +36,7 @@ struct Clue2DBuffers {
   uint32_t* k2 = nullptr;
   int* i2 = nullptr;
   float4* xyw = nullptr;   // tile order: {x, y, weight, point index as int bits}
-  float* rhoS = nullptr;   // tile order: rho
+  uint32_t* rhoKey = nullptr;  // tile order: rho as an order-preserving unsigned key (rho_key)
   int4* reachD = nullptr;  // per 256-point tile: staging / table reach of the density boxes
   int4* reachN = nullptr;  // the same for the nearest-higher boxes
   int* hist = nullptr;

@@ -44,6 +44,7 @@ Clue2DParams make_clue2d_params(float dc, float rhoc, float odf) {
   p.outlierDeltaFactor = odf;
   p.dc2 = sqrt_le_threshold(dc);
   p.dm2 = sqrt_le_threshold(odf * dc);
+  p.dm2up = nextafterf(p.dm2, INFINITY);
   return p;
 }
 
@@ -110,10 +111,10 @@ void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DPara
   k_reach2d<<<(nTiles + kReachThreads - 1) / kReachThreads, kReachThreads, 0, s>>>(
       b.blocks, nTiles, keyS, b.table, b.tableStride, prm.dc, prm.outlierDeltaFactor, b.reachD, b.reachN);
   launch_density2d(b.densityVariant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.table, b.tableStride,
-                   (const int4*)b.reachD, prm.dc, prm.dc2, b.rhoS, io.rho);
+                   (const int4*)b.reachD, prm.dc, prm.dc2, b.rhoKey, io.rho);
   int* parent = reinterpret_cast<int*>(b.k2);
-  launch_nh2d(b.nhVariant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.rhoS, b.table, b.tableStride,
-              (const int4*)b.reachN, b.evOff, prm.dc, prm.rhoc, prm.outlierDeltaFactor, prm.dm2, io.delta,
+  launch_nh2d(b.nhVariant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.rhoKey, b.table, b.tableStride,
+              (const int4*)b.reachN, b.evOff, prm.dc, prm.rhoc, prm.outlierDeltaFactor, prm.dm2, prm.dm2up, io.delta,
               io.nearestHigher, io.isSeed, parent);
   launch_labels(b.blocks, b.nBlocks, parent, io.isSeed, b.N, b.evOff, io.clusterIndex, b.seeds, b.part, s);
 }
@@ -125,10 +126,10 @@ void clue2d_enqueue_step(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2
   int* parent = reinterpret_cast<int*>(b.k2);
   if (which == 0)
     launch_density2d(variant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.table, b.tableStride, (const int4*)b.reachD,
-                     prm.dc, prm.dc2, b.rhoS, io.rho);
+                     prm.dc, prm.dc2, b.rhoKey, io.rho);
   else
-    launch_nh2d(variant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.rhoS, b.table, b.tableStride,
-                (const int4*)b.reachN, b.evOff, prm.dc, prm.rhoc, prm.outlierDeltaFactor, prm.dm2, io.delta,
+    launch_nh2d(variant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.rhoKey, b.table, b.tableStride,
+                (const int4*)b.reachN, b.evOff, prm.dc, prm.rhoc, prm.outlierDeltaFactor, prm.dm2, prm.dm2up, io.delta,
                 io.nearestHigher, io.isSeed, parent);
 }
 
@@ -150,7 +151,7 @@ void clue2d_carve(void* base, int64_t N, int E, int nBlocks, Clue2DBuffers& b) {
   b.k2 = cv.take<uint32_t>(n);
   b.i2 = cv.take<int>(n);
   b.xyw = cv.take<float4>(n);
-  b.rhoS = cv.take<float>(n);
+  b.rhoKey = cv.take<uint32_t>(n);
   b.reachD = cv.take<int4>((size_t)(nBlocks > 0 ? nBlocks : 1) * kTilesPerSort);
   b.reachN = cv.take<int4>((size_t)(nBlocks > 0 ? nBlocks : 1) * kTilesPerSort);
   b.hist = cv.take<int>((size_t)kRadix * (nBlocks > 0 ? nBlocks : 1));
@@ -170,7 +171,7 @@ size_t clue2d_workspace_bytes(int64_t N, int nEvents, int nBlocks) {
   auto add = [&](size_t sz) { bytes += (sz + 255) & ~(size_t)255; };
   for (int k = 0; k < 4; ++k) add(n * 4);                   // k1 i1 k2 i2
   add(n * 16);                                              // xyw
-  add(n * 4);                                               // rhoS
+  add(n * 4);                                               // rhoKey
   for (int k = 0; k < 2; ++k) add((size_t)(nBlocks > 0 ? nBlocks : 1) * kTilesPerSort * sizeof(int4));  // reach
   add((size_t)kRadix * (size_t)(nBlocks > 0 ? nBlocks : 1) * 4);  // hist
   const int64_t maxScan = (int64_t)kRadix * nBlocks > N ? (int64_t)kRadix * nBlocks : N;

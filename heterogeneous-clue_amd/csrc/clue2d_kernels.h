@@ -209,14 +209,36 @@ __device__ __forceinline__ float4 cand(const float4* __restrict__ g, const float
     return g[q];
 }
 
-// Nearest-higher candidates {x, y, rho, index bits}: x, y and the index from the tile-order
-// {x, y, w, index} records, rho from the tile-order rho array the density step writes.
+// rho as an unsigned key in the fp32 order of rho: -0 is taken as +0, and NaN maps to 0, below every
+// number (a point of NaN rho is higher than no other point). The nearest-higher step compares
+// (rho key << 32 | point index) as one 64-bit integer, which is the reference's "higher" relation
+// rho_j > rho_i || (rho_j == rho_i && j > i) (CLUEAlgoKernels.h:84-86) for every pair of points.
+__device__ __forceinline__ uint32_t rho_key(float r) {
+  uint32_t u = (uint32_t)__float_as_int(r);
+  if (u == 0x80000000u) u = 0u;
+  const uint32_t k = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  return r != r ? 0u : k;
+}
+__device__ __forceinline__ float rho_of_key(uint32_t k) {
+  const uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  return k == 0u ? __int_as_float(0x7fc00000) : __int_as_float((int)u);
+}
+// The query side of the relation: a point of NaN rho has no higher point at all.
+__device__ __forceinline__ uint64_t query_key(uint32_t rk, int i) {
+  return rk == 0u ? ~0ull : (((uint64_t)rk << 32) | (uint32_t)i);
+}
+__device__ __forceinline__ uint64_t cand_key(const float4& c) {
+  return ((uint64_t)(uint32_t)__float_as_int(c.w) << 32) | (uint32_t)__float_as_int(c.z);
+}
+
+// Nearest-higher candidates {x, y, index bits, rho key bits}: x, y and the index from the tile-order
+// {x, y, w, index} records, the rho key from the tile-order key array the density step writes.
 struct NhSrc {
   const float4* xyw;
-  const float* rho;
+  const uint32_t* rhoKey;
   __device__ __forceinline__ float4 at(int q) const {
     const float4 a = xyw[q];
-    return make_float4(a.x, a.y, rho[q], a.w);
+    return make_float4(a.x, a.y, a.w, __int_as_float((int)rhoKey[q]));
   }
 };
 template <bool STAGED>
@@ -234,18 +256,18 @@ __device__ __forceinline__ void stage_range_nh(const NhSrc& g, float4* st, int l
   constexpr int kPer = STAGE / kThreads;
   const int n = hi - lo;
   float4 v[kPer];
-  float r[kPer];
+  uint32_t r[kPer];
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
     const int k = u * kThreads + (int)threadIdx.x;
     const int q = lo + (k < n ? k : n - 1);
     v[u] = g.xyw[q];
-    r[u] = g.rho[q];
+    r[u] = g.rhoKey[q];
   }
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
     const int k = u * kThreads + (int)threadIdx.x;
-    if (k < n) st[k] = make_float4(v[u].x, v[u].y, r[u], v[u].w);
+    if (k < n) st[k] = make_float4(v[u].x, v[u].y, v[u].w, __int_as_float((int)r[u]));
   }
 }
 
@@ -343,17 +365,19 @@ __global__ __launch_bounds__(kThreads) void k_density2d(const BlockDesc* __restr
                                                         const float4* __restrict__ xyw,
                                                         const int* __restrict__ table, int64_t stride,
                                                         const int4* __restrict__ reach, float dc, float dc2,
-                                                        float* __restrict__ rhoS, float* __restrict__ rhoOut) {
+                                                        uint32_t* __restrict__ rhoKey, float* __restrict__ rhoOut) {
   using Shape = CompShape<ITEMS>;
   __shared__ float4 s_st[STAGE];
   __shared__ int s_tab[kTabStage];
-  const BlockDesc bd = blocks[blockIdx.x / Shape::kPerSort];
-  const int b0 = bd.begin + (int)(blockIdx.x % Shape::kPerSort) * Shape::kHits;
+  // consecutive compute blocks (overlapping staged ranges) run on one XCD and share its L2
+  const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  const BlockDesc bd = blocks[blk / Shape::kPerSort];
+  const int b0 = bd.begin + (blk % Shape::kPerSort) * Shape::kHits;
   const int b1 = (b0 + Shape::kHits) < bd.end ? (b0 + Shape::kHits) : bd.end;
   if (b0 >= b1) return;  // uniform over the block
   const int* T = table + (int64_t)bd.event * stride;
   int rlo, rhi, tlo, tn;
-  compute_reach<ITEMS>(reach, (int)blockIdx.x, b0, b1, rlo, rhi, tlo, tn);
+  compute_reach<ITEMS>(reach, blk, b0, b1, rlo, rhi, tlo, tn);
   const bool staged = rhi - rlo <= STAGE;
   if (staged) stage_range<STAGE>(xyw, s_st, rlo, rhi);
   const TableView tab{T, s_tab, tlo, tn <= kTabStage};
@@ -392,7 +416,7 @@ __global__ __launch_bounds__(kThreads) void k_density2d(const BlockDesc* __restr
                      : density_run<false>(p, xi, yi, s, t, xyw, s_st, rlo, dc2, rho);
       }
       const int i = __float_as_int(me[r].w);
-      rhoS[p] = rho;
+      rhoKey[p] = rho_key(rho);
       rhoOut[i] = rho;
     }
   }
@@ -406,23 +430,21 @@ __global__ __launch_bounds__(kThreads) void k_density2d(const BlockDesc* __restr
 // unless the previous minimum has the same sqrtf as the final one, no earlier candidate can tie
 // with it and the final minimum is the reference's answer. In the tie case the scan is redone
 // exactly as the reference does it.
+// The running minimum starts at dm2up, the float after dm2, so that "d2 < best2" also enforces
+// d2 <= dm2 until the first update and is implied by it afterwards; a previous minimum equal to
+// dm2up never ties (sqrtf(dm2up) > dm >= sqrtf of any accepted d2).
 template <bool STAGED>
-__device__ __forceinline__ void nh_run(int i, float xi, float yi, float rhoi, int s, int t, const NhSrc& src,
-                                       const float4* st, int lo, float dm2,
-                                       float& best2, int& best, float& prev2, int& prev) {
+__device__ __forceinline__ void nh_run(uint64_t qkey, float xi, float yi, int s, int t, const NhSrc& src,
+                                       const float4* st, int lo, float& best2, int& best, float& prev2) {
   for (int q = s; q < t; ++q) {
     const float4 c = cand<STAGED>(src, st, q, lo);
     const float dx = xi - c.x;
     const float dy = yi - c.y;
     const float d2 = dx * dx + dy * dy;
-    const int j = __float_as_int(c.w);
-    // bitwise forms: lane masks combined without branches
-    const bool higher = (c.z > rhoi) | ((c.z == rhoi) & (j > i));
-    const bool upd = higher & (d2 <= dm2) & (d2 < best2);
+    const bool upd = (cand_key(c) > qkey) & (d2 < best2);
     prev2 = upd ? best2 : prev2;
-    prev = upd ? best : prev;
     best2 = upd ? d2 : best2;
-    best = upd ? j : best;
+    best = upd ? __float_as_int(c.z) : best;
   }
 }
 
@@ -430,25 +452,22 @@ __device__ __forceinline__ void nh_run(int i, float xi, float yi, float rhoi, in
 // nh_run.
 template <bool STAGED>
 struct NhStep {
-  int i;
-  float xi, yi, rhoi, dm2;
+  uint64_t qkey;
+  float xi, yi;
   NhSrc g;
   const float4* st;
   int lo;
   float best2, prev2;
-  int best, prev;
+  int best;
   __device__ __forceinline__ void take(const float4& c, bool valid) {
     const float dx = xi - c.x;
     const float dy = yi - c.y;
     const float d2 = dx * dx + dy * dy;
-    const int j = __float_as_int(c.w);
     // bitwise forms: lane masks combined without branches
-    const bool higher = (c.z > rhoi) | ((c.z == rhoi) & (j > i));
-    const bool upd = valid & higher & (d2 <= dm2) & (d2 < best2);
+    const bool upd = valid & (cand_key(c) > qkey) & (d2 < best2);
     prev2 = upd ? best2 : prev2;
-    prev = upd ? best : prev;
     best2 = upd ? d2 : best2;
-    best = upd ? j : best;
+    best = upd ? __float_as_int(c.z) : best;
   }
   __device__ __forceinline__ void operator()(int q0, int q1, bool two) {
     const float4 c0 = cand<STAGED>(g, st, q0, lo);
@@ -459,48 +478,46 @@ struct NhStep {
 };
 
 template <bool STAGED>
-__device__ __forceinline__ void nh_run_exact(int i, float xi, float yi, float rhoi, int s, int t,
-                                             const NhSrc& src, const float4* st, int lo, float dm2,
-                                             float& delta, int& nh) {
+__device__ __forceinline__ void nh_run_exact(uint64_t qkey, float xi, float yi, int s, int t, const NhSrc& src,
+                                             const float4* st, int lo, float dm2, float& delta, int& nh) {
   for (int q = s; q < t; ++q) {
     const float4 c = cand<STAGED>(src, st, q, lo);
     const float dx = xi - c.x;
     const float dy = yi - c.y;
     const float d2 = dx * dx + dy * dy;
-    const int j = __float_as_int(c.w);
-    if (((c.z > rhoi) || ((c.z == rhoi) && (j > i))) && d2 <= dm2) {
+    if (cand_key(c) > qkey && d2 <= dm2) {
       const float d = sqrtf(d2);
       if (d < delta) {
         delta = d;
-        nh = j;
+        nh = __float_as_int(c.z);
       }
     }
   }
 }
 
 template <bool STAGED>
-__device__ __forceinline__ int nearest_higher_of(int i, float xi, float yi, float rhoi, const int* __restrict__ T,
-                                                 const Runs& runs, const NhSrc& src,
-                                                 const float4* st, int lo, float dm2, float& delta) {
-  NhStep<STAGED> f{i, xi, yi, rhoi, dm2, src, st, lo, INFINITY, INFINITY, -1, -1};
+__device__ __forceinline__ int nearest_higher_of(uint64_t qkey, float xi, float yi, const int* __restrict__ T,
+                                                 const Runs& runs, const NhSrc& src, const float4* st, int lo,
+                                                 float dm2, float dm2up, float& delta) {
+  NhStep<STAGED> f{qkey, xi, yi, src, st, lo, dm2up, INFINITY, -1};
   walk_runs(runs, f);
   float best2 = f.best2, prev2 = f.prev2;
-  int best = f.best, prev = f.prev;
+  int best = f.best;
   for (int k = kMaxCols; k < runs.n; ++k) {
     int s, t;
     run_bounds(T, runs, k, s, t);
-    nh_run<STAGED>(i, xi, yi, rhoi, s, t, src, st, lo, dm2, best2, best, prev2, prev);
+    nh_run<STAGED>(qkey, xi, yi, s, t, src, st, lo, best2, best, prev2);
   }
   delta = FLT_MAX;
   if (best < 0) return -1;
   delta = sqrtf(best2);
-  if (prev < 0 || sqrtf(prev2) != delta) return best;
+  if (sqrtf(prev2) != delta) return best;
   delta = FLT_MAX;
   int nh = -1;
   for (int k = 0; k < runs.n; ++k) {
     int s, t;
     run_bounds(T, runs, k, s, t);
-    nh_run_exact<STAGED>(i, xi, yi, rhoi, s, t, src, st, lo, dm2, delta, nh);
+    nh_run_exact<STAGED>(qkey, xi, yi, s, t, src, st, lo, dm2, delta, nh);
   }
   return nh;
 }
@@ -510,26 +527,28 @@ __device__ __forceinline__ int nearest_higher_of(int i, float xi, float yi, floa
 template <int ITEMS, int STAGE>
 __global__ __launch_bounds__(kThreads) void k_nh2d(const BlockDesc* __restrict__ blocks,
                                                    const uint32_t* __restrict__ keyS,
-                                                   const float4* __restrict__ xyw, const float* __restrict__ rhoS,
+                                                   const float4* __restrict__ xyw, const uint32_t* __restrict__ rhoKey,
                                                    const int* __restrict__ table, int64_t stride,
                                                    const int4* __restrict__ reach, const int* __restrict__ evOff,
                                                    float dc, float rhoc,
-                                                   float odf, float dm2, float* __restrict__ deltaOut,
+                                                   float odf, float dm2, float dm2up, float* __restrict__ deltaOut,
                                                    int32_t* __restrict__ nhOut, int32_t* __restrict__ isSeedOut,
                                                    int* __restrict__ parent) {
   using Shape = CompShape<ITEMS>;
   __shared__ float4 s_st[STAGE];
   __shared__ int s_tab[kTabStage];
-  const BlockDesc bd = blocks[blockIdx.x / Shape::kPerSort];
-  const int b0 = bd.begin + (int)(blockIdx.x % Shape::kPerSort) * Shape::kHits;
+  // consecutive compute blocks (overlapping staged ranges) run on one XCD and share its L2
+  const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  const BlockDesc bd = blocks[blk / Shape::kPerSort];
+  const int b0 = bd.begin + (blk % Shape::kPerSort) * Shape::kHits;
   const int b1 = (b0 + Shape::kHits) < bd.end ? (b0 + Shape::kHits) : bd.end;
   if (b0 >= b1) return;
   const int* T = table + (int64_t)bd.event * stride;
   const int evBase = evOff[bd.event];
   const float dm = odf * dc;
-  const NhSrc src{xyw, rhoS};
+  const NhSrc src{xyw, rhoKey};
   int rlo, rhi, tlo, tn;
-  compute_reach<ITEMS>(reach, (int)blockIdx.x, b0, b1, rlo, rhi, tlo, tn);
+  compute_reach<ITEMS>(reach, blk, b0, b1, rlo, rhi, tlo, tn);
   const bool staged = rhi - rlo <= STAGE;
   if (staged) stage_range_nh<STAGE>(src, s_st, rlo, rhi);
   const TableView tab{T, s_tab, tlo, tn <= kTabStage};
@@ -550,13 +569,15 @@ __global__ __launch_bounds__(kThreads) void k_nh2d(const BlockDesc* __restrict__
   for (int r = 0; r < ITEMS; ++r) {
     const int p = b0 + r * kThreads + (int)threadIdx.x;
     if (p < b1) {
-      const int i = __float_as_int(me[r].w);
-      const float rhoi = me[r].z;
+      const int i = __float_as_int(me[r].z);
+      const uint32_t rk = (uint32_t)__float_as_int(me[r].w);
+      const float rhoi = rho_of_key(rk);
+      const uint64_t qkey = query_key(rk, i);
       float delta;
-      const int nhg = staged
-                          ? nearest_higher_of<true>(i, me[r].x, me[r].y, rhoi, T, runs[r], src, s_st, rlo, dm2, delta)
-                          : nearest_higher_of<false>(i, me[r].x, me[r].y, rhoi, T, runs[r], src, s_st, rlo, dm2,
-                                                     delta);
+      const int nhg = staged ? nearest_higher_of<true>(qkey, me[r].x, me[r].y, T, runs[r], src, s_st, rlo, dm2, dm2up,
+                                                       delta)
+                             : nearest_higher_of<false>(qkey, me[r].x, me[r].y, T, runs[r], src, s_st, rlo, dm2,
+                                                        dm2up, delta);
       deltaOut[i] = delta;
       nhOut[i] = nhg >= 0 ? nhg - evBase : -1;
       const bool seed = (delta > dc) && (rhoi >= rhoc);

@@ -46,12 +46,12 @@ int main(int argc, char** argv) {
   std::vector<int> part(scan_partials(std::max<int64_t>(maxScan, 1)), P);
   std::vector<int> table((size_t)stride * std::max(E, 1), P), tileStart((size_t)(tiles + 1) * std::max(E, 1), P);
   std::vector<float4> xyw(N, float4{NAN, NAN, NAN, NAN});
-  std::vector<float> rhoS(N, NAN);
+  std::vector<uint32_t> rhoKey(N, 0x5A5A5A5Au);
   std::vector<float> rho(N, -7.f), delta(N, -7.f);
   std::vector<int> nh(N, -7), isSeed(N, -7), ci(N, -7);
   int err = 0;
   const float dc = in.dc, rhoc = in.rhoc, odf = in.odf;
-  const float dc2 = sqrt_le(dc), dm2 = sqrt_le(odf * dc);
+  const float dc2 = sqrt_le(dc), dm2 = sqrt_le(odf * dc), dm2up = std::nextafter(dm2, INFINITY);
   if (nb > 0) {
     const int nHist = kRadix * nb;
     Keys2D keys{in.x.data(), in.y.data(), in.layer.data(), k2.data(), &err};
@@ -84,14 +84,14 @@ int main(int argc, char** argv) {
     dispatch(dv, [&](auto I, auto S) {
       emu::launch(CompShape<I>::kPerSort * nb, kThreads, [&] {
         k_density2d<I, S>(blocks.data(), k1.data(), xyw.data(), table.data(), stride, reachD.data(), dc, dc2,
-                          rhoS.data(), rho.data());
+                          rhoKey.data(), rho.data());
       });
     });
     int* parent = reinterpret_cast<int*>(k2.data());
     dispatch(nv, [&](auto I, auto S) {
       emu::launch(CompShape<I>::kPerSort * nb, kThreads, [&] {
-        k_nh2d<I, S>(blocks.data(), k1.data(), xyw.data(), rhoS.data(), table.data(), stride, reachN.data(),
-                     evOff.data(), dc, rhoc, odf, dm2, delta.data(), nh.data(), isSeed.data(), parent);
+        k_nh2d<I, S>(blocks.data(), k1.data(), xyw.data(), rhoKey.data(), table.data(), stride, reachN.data(),
+                     evOff.data(), dc, rhoc, odf, dm2, dm2up, delta.data(), nh.data(), isSeed.data(), parent);
       });
     });
     SeedRankHost seeds(N);
