@@ -343,10 +343,12 @@ class Server:
     def _err(self):
         return self.dev.lib.clue_server_last_error(self.h).decode()
 
-    def submit2d(self, ev: dict) -> int:
+    def submit2d(self, ev: dict, outs: dict | None = None) -> int:
+        """Queue one CLUE2D event; `outs` may give the output arrays (e.g. pinned ones)."""
         ins = [np.ascontiguousarray(ev[k], dtype=_dt(k)) for k in FIELDS_2D_IN]
         n = ins[0].size
-        outs = {k: np.empty(n, _dt(k)) for k in FIELDS_2D_OUT}
+        outs = outs if outs is not None else {k: np.empty(n, _dt(k)) for k in FIELDS_2D_OUT}
+        assert all(outs[k].size == n and outs[k].dtype == _dt(k) and outs[k].flags.c_contiguous for k in FIELDS_2D_OUT)
         rid = self.dev.lib.clue_server_submit2d(self.h, n, *[a.ctypes.data for a in ins],
                                                 *[outs[k].ctypes.data for k in FIELDS_2D_OUT])
         if rid < 0:
@@ -354,10 +356,12 @@ class Server:
         self._live[rid] = (ins, outs)
         return int(rid)
 
-    def submit3d(self, ev: dict) -> int:
+    def submit3d(self, ev: dict, outs: dict | None = None) -> int:
+        """Queue one CLUE3D event; `outs` may give the output arrays (e.g. pinned ones)."""
         ins = [np.ascontiguousarray(ev[k], dtype=_dt(k)) for k in FIELDS_3D_IN]
         n = ins[0].size
-        outs = {k: np.empty(n, _dt(k)) for k in FIELDS_3D_OUT}
+        outs = outs if outs is not None else {k: np.empty(n, _dt(k)) for k in FIELDS_3D_OUT}
+        assert all(outs[k].size == n and outs[k].dtype == _dt(k) and outs[k].flags.c_contiguous for k in FIELDS_3D_OUT)
         rid = self.dev.lib.clue_server_submit3d(self.h, n, *[a.ctypes.data for a in ins],
                                                 *[outs[k].ctypes.data for k in FIELDS_3D_OUT])
         if rid < 0:
@@ -372,6 +376,10 @@ class Server:
             raise ClueError(rc, self._err())
         _, outs = self._live.pop(rid)
         return outs, lat.value
+
+    def outputs(self, rid: int) -> dict:
+        """The output arrays of a request not yet waited for (kept referenced by the server object)."""
+        return self._live[rid][1]
 
     def stats(self):
         b, e, p = C.c_int64(), C.c_int64(), C.c_int64()

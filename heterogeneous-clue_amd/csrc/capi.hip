@@ -181,6 +181,48 @@ int upload_tables(clue_ctx* c, const int64_t* offsets, int E, const BlockDesc* b
   return CLUE_OK;
 }
 
+// The same tables for a call that must not wait for the device: built on the host into a pinned
+// buffer of the context's ring and copied with hipMemcpyAsync on `s`, so they overwrite the shared
+// workspace only after the work enqueued on `s` before them (other streams are ordered by
+// claim_scratch). Waits only when the ring buffer's previous copy has not run yet.
+int upload_tables_async(clue_ctx* c, const int64_t* offsets, int E, const BlockDesc* blocksDev, const int* evOffDev,
+                        hipStream_t s, const int* evBlkDev = nullptr) {
+  const std::vector<BlockDesc> blocks = make_blocks(offsets, E);
+  const size_t bBlk = blocks.size() * sizeof(BlockDesc), bOff = ((size_t)E + 1) * sizeof(int);
+  const size_t oOff = (bBlk + 255) & ~(size_t)255, oEvb = oOff + ((bOff + 255) & ~(size_t)255);
+  const size_t need = oEvb + (evBlkDev ? bOff : 0);
+  TableRing& R = c->tables;
+  const int k = R.next;
+  R.next = (k + 1) % kTableRing;
+  if (R.pending[k]) {
+    HIP_TRY(c, hipEventSynchronize(R.read[k]));
+    R.pending[k] = false;
+  }
+  if (!R.read[k]) HIP_TRY(c, hipEventCreateWithFlags(&R.read[k], hipEventDisableTiming));
+  if (R.bytes[k] < need) {
+    if (R.buf[k]) HIP_TRY(c, hipHostFree(R.buf[k]));
+    R.buf[k] = nullptr;
+    R.bytes[k] = 0;
+    HIP_TRY(c, hipHostMalloc(&R.buf[k], need, hipHostMallocDefault));
+    R.bytes[k] = need;
+  }
+  char* h = static_cast<char*>(R.buf[k]);
+  if (bBlk) std::memcpy(h, blocks.data(), bBlk);
+  int* off32 = reinterpret_cast<int*>(h + oOff);
+  for (int e = 0; e <= E; ++e) off32[e] = (int)offsets[e];
+  if (bBlk)
+    HIP_TRY(c, hipMemcpyAsync(const_cast<BlockDesc*>(blocksDev), h, bBlk, hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(const_cast<int*>(evOffDev), off32, bOff, hipMemcpyHostToDevice, s));
+  if (evBlkDev) {
+    const std::vector<int> eb = make_event_blocks(offsets, E);
+    std::memcpy(h + oEvb, eb.data(), bOff);
+    HIP_TRY(c, hipMemcpyAsync(const_cast<int*>(evBlkDev), h + oEvb, bOff, hipMemcpyHostToDevice, s));
+  }
+  HIP_TRY(c, hipEventRecord(R.read[k], s));
+  R.pending[k] = true;
+  return CLUE_OK;
+}
+
 int nblocks_of(const int64_t* offsets, int E) {
   int64_t nb = 0;
   for (int e = 0; e < E; ++e) nb += (offsets[e + 1] - offsets[e] + kSortBlock - 1) / kSortBlock;
@@ -314,7 +356,6 @@ int run2d(clue_ctx* c, const Clue2DParams& p, int E, const int64_t* offsets, con
   const int64_t N = offsets[E];
   const int nb = nblocks_of(offsets, E);
   const int64_t maxEv = max_event(offsets, E);
-  if (!single) HIP_TRY(c, hipStreamSynchronize(s));  // the block tables below overwrite the shared scratch
   if (c->path2d != CLUE_PATH_GLOBAL) {
     int rc = ensure(c, c->scratchF2d, fused2d_workspace_bytes(N, E, nb));
     if (rc) return rc;
@@ -325,8 +366,7 @@ int run2d(clue_ctx* c, const Clue2DParams& p, int E, const int64_t* offsets, con
     if (single) {
       rc = single_setup(c, (int)N, nb, fb.blocks, fb.evOff, s, fb.evBlk);
     } else {
-      std::vector<BlockDesc> blocks;
-      rc = upload_tables(c, offsets, E, fb.blocks, fb.evOff, blocks, fb.evBlk);
+      rc = upload_tables_async(c, offsets, E, fb.blocks, fb.evOff, s, fb.evBlk);
     }
     if (rc) return rc;
     const bool forced = c->path2d == CLUE_PATH_FUSED;
@@ -352,8 +392,7 @@ int run2d(clue_ctx* c, const Clue2DParams& p, int E, const int64_t* offsets, con
   if (single) {
     rc = single_setup(c, (int)N, nb, b.blocks, b.evOff, s);
   } else {
-    std::vector<BlockDesc> blocks;
-    rc = upload_tables(c, offsets, E, b.blocks, b.evOff, blocks);
+    rc = upload_tables_async(c, offsets, E, b.blocks, b.evOff, s);
   }
   if (rc) return rc;
   clue2d_enqueue(b, io, p, s);
@@ -456,6 +495,10 @@ void clue_destroy(clue_ctx* c) {
   c->allocs.clear();
   for (int k = 0; k < kMarks; ++k)
     if (c->marks[k]) (void)hipEventDestroy(c->marks[k]);
+  for (int k = 0; k < kTableRing; ++k) {
+    if (c->tables.buf[k]) (void)hipHostFree(c->tables.buf[k]);
+    if (c->tables.read[k]) (void)hipEventDestroy(c->tables.read[k]);
+  }
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -770,15 +813,13 @@ int32_t clue3d_run_batch_device(clue_ctx* c, const clue3d_params* p, int32_t E, 
   hipStream_t s = pick(c, stream);
   int rc = claim_scratch(c, s);
   if (rc) return rc;
-  HIP_TRY(c, hipStreamSynchronize(s));
   const int nb = nblocks_of(offsets, E);
   rc = ensure(c, c->scratch3d, clue3d_workspace_bytes(N, E, nb));
   if (rc) return rc;
   Clue3DBuffers b;
   clue3d_carve(c->scratch3d.base, N, E, nb, b);
   b.err = c->dErr;
-  std::vector<BlockDesc> blocks;
-  rc = upload_tables(c, offsets, E, b.blocks, b.evOff, blocks);
+  rc = upload_tables_async(c, offsets, E, b.blocks, b.evOff, s);
   if (rc) return rc;
   Clue3DIO io{z, eta, phi, roz, radius, energy, layer, isSi, rho, df, dl, nhl, nhi, isSeed, ti};
   clue3d_enqueue(b, io, to3d(p), s);

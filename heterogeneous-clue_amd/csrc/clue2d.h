@@ -45,7 +45,8 @@ struct Clue2DBuffers {
   int* tileStart = nullptr;
   int64_t tableStride = 0;
   SeedRank seeds{};
-  int* err = nullptr;
+  int* err = nullptr;  // device error flags: err[event * errStride]
+  int errStride = 0;
   int densityVariant = 0;  // (points per thread, staging) variant of the density kernel (kVariants2D)
   int nhVariant = 0;       // the same for the nearest-higher kernel
 };

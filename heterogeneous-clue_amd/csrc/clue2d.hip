@@ -89,7 +89,7 @@ void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DPara
   if (b.N <= 0 || b.nBlocks <= 0) return;
   const int nHist = kRadix * b.nBlocks;
   // pass 1: yBin (keys computed and recorded in k2)
-  Keys2D keys{io.x, io.y, io.layer, b.k2, b.err};
+  Keys2D keys{io.x, io.y, io.layer, b.k2, b.err, b.errStride};
   k_radix_hist<Keys2D, DigitY><<<b.nBlocks, kThreads, 0, s>>>(b.blocks, keys, b.hist);
   scan_exclusive(b.hist, b.hist, nHist, b.part, s);
   k_radix_scatter<KeyArray, DigitY, EmitFirst>

@@ -21,16 +21,19 @@
 namespace clue {
 namespace g2d {
 
+// Bin key of point i of event ev; an out-of-range layer sets kErrLayerRange in err[ev * errStride]
+// (errStride 0: one flag word for the whole batch).
 struct Keys2D {
   const float* x;
   const float* y;
   const int32_t* layer;
   uint32_t* keyOut;
   int* err;
-  __device__ __forceinline__ uint32_t operator()(int i) const {
+  int errStride;
+  __device__ __forceinline__ uint32_t operator()(int i, int ev) const {
     int L = layer[i];
     if (L < 0 || L >= k2dLayers) {
-      atomicOr(err, kErrLayerRange);
+      atomicOr(err + ev * errStride, kErrLayerRange);
       L = L < 0 ? 0 : k2dLayers - 1;
     }
     const uint32_t k = (uint32_t)(L * k2dBinsPerLayer + xbin2d(x[i]) * k2dRows + ybin2d(y[i]));

@@ -42,7 +42,8 @@ struct Clue3DBuffers {
   int* tileStart = nullptr;
   int64_t tableStride = 0;
   SeedRank seeds{};
-  int* err = nullptr;
+  int* err = nullptr;  // device error flags: err[event * errStride]
+  int errStride = 0;
   int densityVariant = 0;  // kernel shapes (points per thread, staged points) of the density and
   int nhVariant = 0;       // nearest-higher steps; see clue3d_variant_shape
 };

@@ -56,7 +56,7 @@ void nh_step(const Clue3DBuffers& b, const Clue3DIO& io, const Clue3DParams& prm
 void clue3d_enqueue(const Clue3DBuffers& b, const Clue3DIO& io, const Clue3DParams& prm, hipStream_t s) {
   if (b.N <= 0 || b.nBlocks <= 0) return;
   const int nHist = kRadix * b.nBlocks;
-  Keys3D keys{io.eta, io.phi, io.layer, b.k2, b.err};
+  Keys3D keys{io.eta, io.phi, io.layer, b.k2, b.err, b.errStride};
   k_radix_hist<Keys3D, DigitPhi><<<b.nBlocks, kThreads, 0, s>>>(b.blocks, keys, b.hist);
   scan_exclusive(b.hist, b.hist, nHist, b.part, s);
   k_radix_scatter<KeyArray, DigitPhi, EmitFirst>

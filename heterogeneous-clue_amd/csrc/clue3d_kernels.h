@@ -23,21 +23,24 @@ namespace g3d {
 __device__ __forceinline__ int clamp_layer3d(int L) { return L < 0 ? 0 : (L >= k3dLayers ? k3dLayers - 1 : L); }
 __device__ __forceinline__ int clamp_phibin(int p) { return p < 0 ? 0 : (p > k3dPhi ? k3dPhi : p); }
 
+// Bin key of cluster i of event ev; range errors set bits in err[ev * errStride] (errStride 0: one
+// flag word for the whole batch).
 struct Keys3D {
   const float* eta;
   const float* phi;
   const int32_t* layer;
   uint32_t* keyOut;
   int* err;
-  __device__ __forceinline__ uint32_t operator()(int i) const {
+  int errStride;
+  __device__ __forceinline__ uint32_t operator()(int i, int ev) const {
     int L = layer[i];
     if (L < 0 || L >= k3dLayers) {
-      atomicOr(err, kErrLayerRange);
+      atomicOr(err + ev * errStride, kErrLayerRange);
       L = clamp_layer3d(L);
     }
     int pb = phi_bin3d(phi[i]);
     if (pb < 0 || pb > k3dPhi) {
-      atomicOr(err, kErrBinRange);
+      atomicOr(err + ev * errStride, kErrBinRange);
       pb = clamp_phibin(pb);
     }
     // TICLLayerTileT::globalBin = phiBin + etaBin * 126 (TICLLayerTile.h:52); a phiBin of 126 is

@@ -20,6 +20,19 @@ struct Workspace {
   size_t bytes = 0;
 };
 
+// Pinned host buffers that the asynchronous batch calls fill with a batch's block table and
+// offsets before copying them to the device on the caller's stream. A buffer is reused once the
+// copy that last read it has run (its event), so kTableRing calls can be in flight before a call
+// waits for the device.
+constexpr int kTableRing = 4;
+struct TableRing {
+  void* buf[kTableRing] = {};
+  size_t bytes[kTableRing] = {};
+  hipEvent_t read[kTableRing] = {};
+  bool pending[kTableRing] = {};
+  int next = 0;
+};
+
 }  // namespace detail
 }  // namespace clue
 
@@ -33,6 +46,7 @@ struct clue_ctx {
   hipEvent_t marks[clue::detail::kMarks] = {};
   clue::detail::Workspace scratch2d, scratch3d, hostio;  // run_* workspaces and host-path I/O buffers
   clue::detail::Workspace scratchF2d;                     // run_* workspace of the fused CLUE2D path
+  clue::detail::TableRing tables;                         // pinned staging of batch block tables
   int path2d = CLUE_PATH_GLOBAL;  // CLUE_PATH_AUTO resolves to the global-sort path
   int cus = 256;                                          // compute units of the device
   hipStream_t scratchStream = nullptr;                    // stream of the last run_* that used the scratch

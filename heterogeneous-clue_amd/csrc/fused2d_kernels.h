@@ -28,7 +28,7 @@ constexpr int align16(int b) { return (b + 15) / 16 * 16; }
 struct LayerKey {
   const int32_t* layer;
   int* err;
-  __device__ __forceinline__ uint32_t operator()(int i) const {
+  __device__ __forceinline__ uint32_t operator()(int i, int) const {
     int L = layer[i];
     if (L < 0 || L >= k2dLayers) {
       atomicOr(err, kErrLayerRange);

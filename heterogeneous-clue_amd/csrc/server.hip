@@ -2,14 +2,18 @@
 //
 // Requests (one event each, caller-owned host buffers) are queued by clue_server_submit*. A worker
 // thread owns a fixed set of slots; a slot is one batch in flight with its own HIP stream, pinned
-// staging memory and device workspace, all allocated once at creation against the context's HBM
-// budget and kept resident. Whenever a slot is free and requests are queued, the worker takes the
-// queue head in arrival order (as many as fit the slot: events per batch, points per dimension),
-// packs them into the slot's pinned staging, and enqueues copy-in, the batched kernels and
-// copy-out on the slot's stream. Requests that arrive meanwhile wait for the next free slot, so
-// with two or more slots a new batch starts while earlier ones still run and their work overlaps
-// on the device. When a slot's completion event fires the worker copies each result to its
-// caller's buffers and timestamps it; latency = completion time - submission time.
+// staging memory and one device arena (SoA inputs and outputs, workspaces and per-event error words
+// of its batch), kept resident. The arenas share the context's HBM budget: each slot may hold up to
+// budget / slots bytes (without a budget, what its largest batch of max_events events and
+// max_points points per dimension needs), and an arena grows to that share only when a batch needs
+// it. Whenever a slot is free and requests are queued, the worker admits the queue head in arrival
+// order while the batch's device bytes fit the slot's share (and its event and point counts fit the
+// staging), packs the batch into the slot's pinned staging, and enqueues copy-in, the batched
+// kernels and copy-out on the slot's stream. Requests that arrive meanwhile wait for the next free
+// slot, so with two or more slots a new batch starts while earlier ones still run and their work
+// overlaps on the device. When a slot's completion event fires the worker copies each result to its
+// caller's buffers, reads the event's device error flags, and timestamps it; latency = completion
+// time - submission time.
 //
 // Every event's arithmetic depends only on its own points (blocks never straddle events), so a
 // result is bit-identical whether the event ran alone or in any mix.
@@ -140,8 +144,8 @@ constexpr int kIn2 = 4, kOut2 = 5, kIn3 = 8, kOut3 = 7;
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
-  Workspace ws2, ws3, io2, io3;  // device: batch workspaces and I/O SoA (capacity-sized)
-  char* stage = nullptr;         // pinned host staging
+  Workspace arena;        // device: the batch's SoA, workspaces and error words (batch_layout)
+  char* stage = nullptr;  // pinned host staging
   size_t stageBytes = 0;
   std::vector<Req*> reqs2, reqs3;
   std::vector<int64_t> offs2, offs3;
@@ -154,8 +158,9 @@ struct clue_server {
   clue_ctx* ctx = nullptr;
   Clue2DParams p2{};
   Clue3DParams p3{};
-  int64_t cap2 = 0, cap3 = 0;
+  int64_t cap2 = 0, cap3 = 0;  // points per batch and dimension (pinned staging capacity)
   int capEvents = 1;
+  size_t share = 0;            // device bytes one slot's batch may use
   std::vector<Slot> slots;
   std::mutex m;
   std::condition_variable cvWork, cvDone;
@@ -174,9 +179,9 @@ namespace {
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Pinned staging layout of a slot: 2D inputs and outputs, 3D inputs and outputs (field-major,
-// capacity-sized), then block tables and offsets for both dimensions.
+// capacity-sized), then block tables and offsets for both dimensions and the per-event error words.
 struct StageLayout {
-  size_t in2, out2, in3, out3, blk2, off2, blk3, off3, total;
+  size_t in2, out2, in3, out3, blk2, off2, blk3, off3, err, total;
 };
 
 StageLayout stage_layout(int64_t cap2, int64_t cap3, int capEvents) {
@@ -200,6 +205,37 @@ StageLayout stage_layout(int64_t cap2, int64_t cap3, int capEvents) {
   p += align256(maxBlk3 * sizeof(BlockDesc));
   L.off3 = p;
   p += align256((size_t)(capEvents + 1) * 4);
+  L.err = p;
+  p += align256((size_t)capEvents * 2 * 4);
+  L.total = p;
+  return L;
+}
+
+int64_t sort_blocks(int64_t n) { return (n + kSortBlock - 1) / kSortBlock; }
+
+// Device layout of one batch in a slot's arena: the error words of its 2D and 3D events, then per
+// dimension the SoA inputs and outputs (field-major, f bytes per field) and the workspace.
+struct BatchLayout {
+  size_t err2, err3, in2, ws2, in3, ws3, f2, f3, total;
+};
+
+BatchLayout batch_layout(int64_t N2, int E2, int64_t nb2, int64_t N3, int E3, int64_t nb3) {
+  BatchLayout L;
+  size_t p = 0;
+  L.err2 = p;
+  p += align256((size_t)(E2 > 0 ? E2 : 1) * 4);
+  L.err3 = p;
+  p += align256((size_t)(E3 > 0 ? E3 : 1) * 4);
+  L.f2 = align256((size_t)N2 * 4);
+  L.in2 = p;
+  p += L.f2 * (kIn2 + kOut2);
+  L.ws2 = p;
+  if (E2 > 0) p += align256(clue2d_workspace_bytes(N2, E2, (int)nb2));
+  L.f3 = align256((size_t)N3 * 4);
+  L.in3 = p;
+  p += L.f3 * (kIn3 + kOut3);
+  L.ws3 = p;
+  if (E3 > 0) p += align256(clue3d_workspace_bytes(N3, E3, (int)nb3));
   L.total = p;
   return L;
 }
@@ -214,13 +250,13 @@ int server_fail(clue_server* s, int code, const std::string& msg) {
 // pinned caller buffers by DMA directly; pageable ones through the pinned staging (parallel host
 // copies) and one DMA per contiguous staged run.
 hipError_t copy_in(clue_server* s, Slot& sl, const std::vector<Req*>& reqs, const std::vector<int64_t>& offs,
-                   int nf, char* stage, char* dev, size_t stride) {
+                   int nf, char* stage, size_t sstride, char* dev, size_t stride) {
   bool staged = false;
   for (size_t e = 0; e < reqs.size(); ++e)
     if (!reqs[e]->pinnedIn) {
       staged = true;
       for (int k = 0; k < nf; ++k)
-        s->copies->add(stage + k * stride + (size_t)offs[e] * 4, reqs[e]->in[k], (size_t)reqs[e]->n * 4);
+        s->copies->add(stage + k * sstride + (size_t)offs[e] * 4, reqs[e]->in[k], (size_t)reqs[e]->n * 4);
     }
   if (staged) s->copies->run_all();
   for (int k = 0; k < nf; ++k) {
@@ -236,7 +272,8 @@ hipError_t copy_in(clue_server* s, Slot& sl, const std::vector<Req*>& reqs, cons
       size_t f = e;
       while (f < reqs.size() && !reqs[f]->pinnedIn) ++f;  // a run of staged events
       const size_t a = (size_t)offs[e] * 4, b = (size_t)offs[f] * 4;
-      hipError_t r = hipMemcpyAsync(dev + k * stride + a, stage + k * stride + a, b - a, hipMemcpyHostToDevice, sl.stream);
+      hipError_t r = hipMemcpyAsync(dev + k * stride + a, stage + k * sstride + a, b - a, hipMemcpyHostToDevice,
+                                    sl.stream);
       if (r != hipSuccess) return r;
       e = f;
     }
@@ -247,7 +284,7 @@ hipError_t copy_in(clue_server* s, Slot& sl, const std::vector<Req*>& reqs, cons
 // Outputs of a batch: into pinned caller buffers directly, otherwise into the staging (copied to
 // the caller by finish_slot).
 hipError_t copy_out(Slot& sl, const std::vector<Req*>& reqs, const std::vector<int64_t>& offs, int nf, char* stage,
-                    char* dev, size_t stride) {
+                    size_t sstride, char* dev, size_t stride) {
   for (int k = 0; k < nf; ++k) {
     size_t e = 0;
     while (e < reqs.size()) {
@@ -261,7 +298,8 @@ hipError_t copy_out(Slot& sl, const std::vector<Req*>& reqs, const std::vector<i
       size_t f = e;
       while (f < reqs.size() && !reqs[f]->pinnedOut) ++f;
       const size_t a = (size_t)offs[e] * 4, b = (size_t)offs[f] * 4;
-      hipError_t r = hipMemcpyAsync(stage + k * stride + a, dev + k * stride + a, b - a, hipMemcpyDeviceToHost, sl.stream);
+      hipError_t r = hipMemcpyAsync(stage + k * sstride + a, dev + k * stride + a, b - a, hipMemcpyDeviceToHost,
+                                    sl.stream);
       if (r != hipSuccess) return r;
       e = f;
     }
@@ -273,73 +311,106 @@ hipError_t copy_out(Slot& sl, const std::vector<Req*>& reqs, const std::vector<i
 int launch_slot(clue_server* s, Slot& sl) {
   clue_ctx* c = s->ctx;
   const StageLayout L = stage_layout(s->cap2, s->cap3, s->capEvents);
-  const size_t f2 = align256((size_t)s->cap2 * 4), f3 = align256((size_t)s->cap3 * 4);
+  const size_t sf2 = align256((size_t)s->cap2 * 4), sf3 = align256((size_t)s->cap3 * 4);
   char* st = sl.stage;
+  const int E2 = (int)sl.reqs2.size(), E3 = (int)sl.reqs3.size();
+  sl.offs2.assign((size_t)E2 + 1, 0);
+  for (int e = 0; e < E2; ++e) sl.offs2[e + 1] = sl.offs2[e] + sl.reqs2[e]->n;
+  sl.offs3.assign((size_t)E3 + 1, 0);
+  for (int e = 0; e < E3; ++e) sl.offs3[e + 1] = sl.offs3[e] + sl.reqs3[e]->n;
+  int64_t nb2 = 0, nb3 = 0;
+  for (int e = 0; e < E2; ++e) nb2 += sort_blocks(sl.reqs2[e]->n);
+  for (int e = 0; e < E3; ++e) nb3 += sort_blocks(sl.reqs3[e]->n);
+  const BatchLayout B = batch_layout(sl.offs2[E2], E2, nb2, sl.offs3[E3], E3, nb3);
+  if (B.total > sl.arena.bytes) {
+    // the slot is idle (its previous batch finished), so its arena can be replaced at once; it grows
+    // by at least half so that a few early batches settle its size within the slot's share
+    size_t want = sl.arena.bytes + sl.arena.bytes / 2;
+    want = want > B.total ? want : B.total;
+    want = want < s->share ? want : s->share;
+    if (sl.arena.base) {
+      const int rc = dev_free(c, sl.arena.base);
+      sl.arena.base = nullptr;
+      sl.arena.bytes = 0;
+      if (rc) return rc;
+    }
+    const int rc = dev_alloc(c, &sl.arena.base, want);
+    if (rc) return rc;
+    sl.arena.bytes = want;
+  }
+  char* dev = reinterpret_cast<char*>(sl.arena.base);
+  int* err2 = reinterpret_cast<int*>(dev + B.err2);
+  int* err3 = reinterpret_cast<int*>(dev + B.err3);
+  fill_i32(err2, 0, (int64_t)(B.err3 - B.err2) / 4, sl.stream);
+  fill_i32(err3, 0, (int64_t)(B.in2 - B.err3) / 4, sl.stream);
   // ---- 2D
-  if (!sl.reqs2.empty()) {
-    const int E = (int)sl.reqs2.size();
-    sl.offs2.assign((size_t)E + 1, 0);
-    for (int e = 0; e < E; ++e) sl.offs2[e + 1] = sl.offs2[e] + sl.reqs2[e]->n;
-    const int64_t N = sl.offs2[E];
-    char* din = reinterpret_cast<char*>(sl.io2.base);
-    HIP_TRY(c, copy_in(s, sl, sl.reqs2, sl.offs2, kIn2, st + L.in2, din, f2));
-    const int nb = [&] {
-      int64_t b = 0;
-      for (int e = 0; e < E; ++e) b += (sl.offs2[e + 1] - sl.offs2[e] + kSortBlock - 1) / kSortBlock;
-      return (int)b;
-    }();
+  if (E2 > 0) {
+    const int64_t N = sl.offs2[E2];
+    char* din = dev + B.in2;
+    HIP_TRY(c, copy_in(s, sl, sl.reqs2, sl.offs2, kIn2, st + L.in2, sf2, din, B.f2));
     Clue2DBuffers b;
-    clue2d_carve(sl.ws2.base, N, E, nb, b);
-    b.err = c->dErr;
-    const std::vector<BlockDesc> blocks = make_blocks(sl.offs2.data(), E);
+    clue2d_carve(dev + B.ws2, N, E2, (int)nb2, b);
+    b.err = err2;
+    b.errStride = 1;
+    const std::vector<BlockDesc> blocks = make_blocks(sl.offs2.data(), E2);
     std::memcpy(st + L.blk2, blocks.data(), blocks.size() * sizeof(BlockDesc));
     int* off32 = reinterpret_cast<int*>(st + L.off2);
-    for (int e = 0; e <= E; ++e) off32[e] = (int)sl.offs2[e];
+    for (int e = 0; e <= E2; ++e) off32[e] = (int)sl.offs2[e];
     HIP_TRY(c, hipMemcpyAsync(const_cast<BlockDesc*>(b.blocks), st + L.blk2, blocks.size() * sizeof(BlockDesc),
                               hipMemcpyHostToDevice, sl.stream));
-    HIP_TRY(c, hipMemcpyAsync(const_cast<int*>(b.evOff), off32, (size_t)(E + 1) * 4, hipMemcpyHostToDevice, sl.stream));
-    char* dout = din + kIn2 * f2;
-    Clue2DIO io{reinterpret_cast<float*>(din), reinterpret_cast<float*>(din + f2),
-                reinterpret_cast<int32_t*>(din + 2 * f2), reinterpret_cast<float*>(din + 3 * f2),
-                reinterpret_cast<float*>(dout), reinterpret_cast<float*>(dout + f2),
-                reinterpret_cast<int32_t*>(dout + 2 * f2), reinterpret_cast<int32_t*>(dout + 3 * f2),
-                reinterpret_cast<int32_t*>(dout + 4 * f2)};
+    HIP_TRY(c, hipMemcpyAsync(const_cast<int*>(b.evOff), off32, (size_t)(E2 + 1) * 4, hipMemcpyHostToDevice,
+                              sl.stream));
+    char* dout = din + kIn2 * B.f2;
+    const size_t f = B.f2;
+    Clue2DIO io{reinterpret_cast<float*>(din), reinterpret_cast<float*>(din + f),
+                reinterpret_cast<int32_t*>(din + 2 * f), reinterpret_cast<float*>(din + 3 * f),
+                reinterpret_cast<float*>(dout), reinterpret_cast<float*>(dout + f),
+                reinterpret_cast<int32_t*>(dout + 2 * f), reinterpret_cast<int32_t*>(dout + 3 * f),
+                reinterpret_cast<int32_t*>(dout + 4 * f)};
     clue2d_enqueue(b, io, s->p2, sl.stream);
     HIP_TRY(c, hipGetLastError());
-    HIP_TRY(c, copy_out(sl, sl.reqs2, sl.offs2, kOut2, st + L.out2, dout, f2));
+    HIP_TRY(c, copy_out(sl, sl.reqs2, sl.offs2, kOut2, st + L.out2, sf2, dout, B.f2));
   }
   // ---- 3D
-  if (!sl.reqs3.empty()) {
-    const int E = (int)sl.reqs3.size();
-    sl.offs3.assign((size_t)E + 1, 0);
-    for (int e = 0; e < E; ++e) sl.offs3[e + 1] = sl.offs3[e] + sl.reqs3[e]->n;
-    const int64_t N = sl.offs3[E];
-    char* din = reinterpret_cast<char*>(sl.io3.base);
-    HIP_TRY(c, copy_in(s, sl, sl.reqs3, sl.offs3, kIn3, st + L.in3, din, f3));
-    int64_t nb64 = 0;
-    for (int e = 0; e < E; ++e) nb64 += (sl.offs3[e + 1] - sl.offs3[e] + kSortBlock - 1) / kSortBlock;
-    const int nb = (int)nb64;
+  if (E3 > 0) {
+    const int64_t N = sl.offs3[E3];
+    char* din = dev + B.in3;
+    HIP_TRY(c, copy_in(s, sl, sl.reqs3, sl.offs3, kIn3, st + L.in3, sf3, din, B.f3));
     Clue3DBuffers b;
-    clue3d_carve(sl.ws3.base, N, E, nb, b);
-    b.err = c->dErr;
-    const std::vector<BlockDesc> blocks = make_blocks(sl.offs3.data(), E);
+    clue3d_carve(dev + B.ws3, N, E3, (int)nb3, b);
+    b.err = err3;
+    b.errStride = 1;
+    const std::vector<BlockDesc> blocks = make_blocks(sl.offs3.data(), E3);
     std::memcpy(st + L.blk3, blocks.data(), blocks.size() * sizeof(BlockDesc));
     int* off32 = reinterpret_cast<int*>(st + L.off3);
-    for (int e = 0; e <= E; ++e) off32[e] = (int)sl.offs3[e];
+    for (int e = 0; e <= E3; ++e) off32[e] = (int)sl.offs3[e];
     HIP_TRY(c, hipMemcpyAsync(const_cast<BlockDesc*>(b.blocks), st + L.blk3, blocks.size() * sizeof(BlockDesc),
                               hipMemcpyHostToDevice, sl.stream));
-    HIP_TRY(c, hipMemcpyAsync(const_cast<int*>(b.evOff), off32, (size_t)(E + 1) * 4, hipMemcpyHostToDevice, sl.stream));
-    char* dout = din + kIn3 * f3;
-    auto F = [&](char* base, int k) { return reinterpret_cast<float*>(base + k * f3); };
-    auto I = [&](char* base, int k) { return reinterpret_cast<int32_t*>(base + k * f3); };
+    HIP_TRY(c, hipMemcpyAsync(const_cast<int*>(b.evOff), off32, (size_t)(E3 + 1) * 4, hipMemcpyHostToDevice,
+                              sl.stream));
+    char* dout = din + kIn3 * B.f3;
+    auto F = [&](char* base, int k) { return reinterpret_cast<float*>(base + k * B.f3); };
+    auto I = [&](char* base, int k) { return reinterpret_cast<int32_t*>(base + k * B.f3); };
     Clue3DIO io{F(din, 0), F(din, 1), F(din, 2), F(din, 3), F(din, 4), F(din, 5), I(din, 6), I(din, 7),
                 F(dout, 0), F(dout, 1), I(dout, 2), I(dout, 3), I(dout, 4), I(dout, 5), I(dout, 6)};
     clue3d_enqueue(b, io, s->p3, sl.stream);
     HIP_TRY(c, hipGetLastError());
-    HIP_TRY(c, copy_out(sl, sl.reqs3, sl.offs3, kOut3, st + L.out3, dout, f3));
+    HIP_TRY(c, copy_out(sl, sl.reqs3, sl.offs3, kOut3, st + L.out3, sf3, dout, B.f3));
   }
+  // the events' error words come back with the results
+  int* herr = reinterpret_cast<int*>(st + L.err);
+  if (E2 > 0) HIP_TRY(c, hipMemcpyAsync(herr, err2, (size_t)E2 * 4, hipMemcpyDeviceToHost, sl.stream));
+  if (E3 > 0)
+    HIP_TRY(c, hipMemcpyAsync(herr + s->capEvents, err3, (size_t)E3 * 4, hipMemcpyDeviceToHost, sl.stream));
   HIP_TRY(c, hipEventRecord(sl.done, sl.stream));
   return CLUE_OK;
+}
+
+// Status of an event from its device error word (0: none).
+int event_status(int flags) {
+  if (flags & kErrLayerRange) return CLUE_ERR_LAYER;
+  if (flags & kErrBinRange) return CLUE_ERR_BIN;
+  return flags ? CLUE_ERR_HIP : CLUE_OK;
 }
 
 // Copy the results of a finished slot to the callers' buffers and mark the requests done.
@@ -360,15 +431,22 @@ void finish_slot(clue_server* s, Slot& sl, int status) {
     s->copies->run_all();
   }
   const auto now = Clock::now();
+  const int* herr = reinterpret_cast<const int*>(sl.stage + L.err);
   {
     std::lock_guard<std::mutex> g(s->m);
-    for (Req* r : sl.reqs2) {
+    for (size_t e = 0; e < sl.reqs2.size(); ++e) {
+      Req* r = sl.reqs2[e];
+      const int st = status == CLUE_OK ? event_status(herr[e]) : status;
       r->tDone = now;
-      r->state = status == CLUE_OK ? 2 : status;
+      r->state = st == CLUE_OK ? 2 : st;
+      if (st != CLUE_OK && status == CLUE_OK) s->err = "event flagged by the device: input outside the tile array";
     }
-    for (Req* r : sl.reqs3) {
+    for (size_t e = 0; e < sl.reqs3.size(); ++e) {
+      Req* r = sl.reqs3[e];
+      const int st = status == CLUE_OK ? event_status(herr[s->capEvents + e]) : status;
       r->tDone = now;
-      r->state = status == CLUE_OK ? 2 : status;
+      r->state = st == CLUE_OK ? 2 : st;
+      if (st != CLUE_OK && status == CLUE_OK) s->err = "event flagged by the device: input outside the tile array";
     }
   }
   sl.reqs2.clear();
@@ -394,24 +472,31 @@ void worker_loop(clue_server* s) {
         active = true;
       }
     }
-    // admissions: the queue head, in arrival order, into a free slot
+    // admissions: the queue head, in arrival order, into a free slot while the batch's device bytes
+    // fit the slot's share of the HBM budget and its points and events fit the staging
     Slot* free = nullptr;
-    for (auto& sl : s->slots)
-      if (!sl.busy) {
-        free = &sl;
-        break;
-      }
+    bool anyBusy = false;
+    for (auto& sl : s->slots) {
+      anyBusy = anyBusy || sl.busy;
+      if (!sl.busy && !free) free = &sl;
+    }
     if (free) {
       std::unique_lock<std::mutex> lk(s->m);
-      if (s->stop && s->queue.empty()) break;
-      int64_t used2 = 0, used3 = 0;
+      if (s->stop && s->queue.empty() && !anyBusy) break;  // drained: every batch has been finished
+      int64_t used2 = 0, used3 = 0, nb2 = 0, nb3 = 0;
       while (!s->queue.empty()) {
         Req* r = s->queue.front();
-        std::vector<Req*>& dst = r->dim == 2 ? free->reqs2 : free->reqs3;
-        int64_t& used = r->dim == 2 ? used2 : used3;
-        const int64_t cap = r->dim == 2 ? s->cap2 : s->cap3;
-        if ((int)(free->reqs2.size() + free->reqs3.size()) >= s->capEvents || used + r->n > cap) break;
-        used += r->n;
+        const bool d2 = r->dim == 2;
+        std::vector<Req*>& dst = d2 ? free->reqs2 : free->reqs3;
+        const int64_t cap = d2 ? s->cap2 : s->cap3;
+        if ((int)(free->reqs2.size() + free->reqs3.size()) >= s->capEvents || (d2 ? used2 : used3) + r->n > cap) break;
+        const int64_t nb = sort_blocks(r->n);
+        const BatchLayout B =
+            d2 ? batch_layout(used2 + r->n, (int)free->reqs2.size() + 1, nb2 + nb, used3, (int)free->reqs3.size(), nb3)
+               : batch_layout(used2, (int)free->reqs2.size(), nb2, used3 + r->n, (int)free->reqs3.size() + 1, nb3 + nb);
+        if (B.total > s->share) break;
+        (d2 ? used2 : used3) += r->n;
+        (d2 ? nb2 : nb3) += nb;
         dst.push_back(r);
         r->state = 1;
         s->queue.pop_front();
@@ -435,10 +520,12 @@ void worker_loop(clue_server* s) {
       }
     }
     if (active) {
-      std::this_thread::sleep_for(std::chrono::microseconds(10));
+      // a batch is on the device: poll its completion without sleeping (a timed sleep would add the
+      // kernel timer's slack to every event's latency), letting other threads run in between
+      std::this_thread::yield();
     } else {
       std::unique_lock<std::mutex> lk(s->m);
-      if (s->stop && s->queue.empty()) break;
+      if (s->stop && s->queue.empty()) break;  // no slot is busy here
       s->cvWork.wait_for(lk, std::chrono::milliseconds(1), [&] { return s->stop || !s->queue.empty(); });
     }
   }
@@ -453,6 +540,9 @@ int64_t submit(clue_server* s, int dim, int32_t n, const void* const* in, void* 
     if (n > 0 && !out[k]) return server_fail(s, CLUE_ERR_ARG, "null output buffer");
   if (n > (dim == 2 ? s->cap2 : s->cap3))
     return server_fail(s, CLUE_ERR_BUDGET, "event larger than the server's per-batch capacity");
+  const BatchLayout B = dim == 2 ? batch_layout(n, 1, sort_blocks(n), 0, 0, 0) : batch_layout(0, 0, 0, n, 1, sort_blocks(n));
+  if (B.total > s->share)
+    return server_fail(s, CLUE_ERR_BUDGET, "event needs more device memory than a slot's share of the HBM budget");
   // Layers are checked here so that a bad event is rejected alone instead of failing its batch.
   const int32_t* layer = static_cast<const int32_t*>(in[dim == 2 ? 2 : 6]);
   const int nLayers = dim == 2 ? k2dLayers : k3dLayers;
@@ -508,15 +598,22 @@ clue_server* clue_server_create(clue_ctx* c, const clue2d_params* p2, const clue
   s->capEvents = max_events;
   s->slots.resize((size_t)slots);
   const StageLayout L = stage_layout(s->cap2, s->cap3, s->capEvents);
-  const int nb2 = (int)(s->cap2 / kSortBlock + s->capEvents + 1), nb3 = (int)(s->cap3 / kSortBlock + s->capEvents + 1);
-  bool ok = true;
+  // the bytes of the largest batch the capacities allow: max_events events holding max_points points
+  // per dimension (each event adds at most one sort block beyond its points' share)
+  const BatchLayout full = batch_layout(s->cap2, s->capEvents, s->cap2 / kSortBlock + s->capEvents + 1, s->cap3,
+                                        s->capEvents, s->cap3 / kSortBlock + s->capEvents + 1);
+  if (c->budget) {
+    const uint64_t left = c->budget > c->inUse ? c->budget - c->inUse : 0;
+    s->share = (size_t)(left / (uint64_t)slots) & ~(size_t)255;
+  } else {
+    s->share = full.total;
+  }
+  const size_t first = full.total < s->share ? full.total : s->share;
+  bool ok = s->share > 0;
   for (auto& sl : s->slots) {
     ok = ok && hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess;
-    ok = ok && ensure(c, sl.ws2, clue2d_workspace_bytes(s->cap2, s->capEvents, nb2)) == CLUE_OK;
-    ok = ok && ensure(c, sl.ws3, clue3d_workspace_bytes(s->cap3, s->capEvents, nb3)) == CLUE_OK;
-    ok = ok && ensure(c, sl.io2, align256((size_t)s->cap2 * 4) * (kIn2 + kOut2)) == CLUE_OK;
-    ok = ok && ensure(c, sl.io3, align256((size_t)s->cap3 * 4) * (kIn3 + kOut3)) == CLUE_OK;
+    ok = ok && ensure(c, sl.arena, first) == CLUE_OK;
     if (ok) {
       void* p = nullptr;
       ok = hipHostMalloc(&p, L.total, hipHostMallocDefault) == hipSuccess;
@@ -528,8 +625,7 @@ clue_server* clue_server_create(clue_ctx* c, const clue2d_params* p2, const clue
   if (!ok) {
     for (auto& sl : s->slots) {
       if (sl.stage) (void)hipHostFree(sl.stage);
-      for (Workspace* w : {&sl.ws2, &sl.ws3, &sl.io2, &sl.io3})
-        if (w->base) dev_free(c, w->base);
+      if (sl.arena.base) dev_free(c, sl.arena.base);
       if (sl.done) (void)hipEventDestroy(sl.done);
       if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
@@ -553,8 +649,7 @@ void clue_server_destroy(clue_server* s) {
   for (auto& sl : s->slots) {
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
     if (sl.stage) (void)hipHostFree(sl.stage);
-    for (Workspace* w : {&sl.ws2, &sl.ws3, &sl.io2, &sl.io3})
-      if (w->base) dev_free(s->ctx, w->base);
+    if (sl.arena.base) dev_free(s->ctx, sl.arena.base);
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
   }

@@ -55,10 +55,10 @@ struct Digit {
   __device__ __forceinline__ static int get(uint32_t k) { return (int)((k / (uint32_t)DIV) % (uint32_t)MOD); }
 };
 
-// Reads the key of input position i (and may record it as a side effect).
+// Reads the key of input position i of event ev (and may record it as a side effect).
 struct KeyArray {
   const uint32_t* key;
-  __device__ __forceinline__ uint32_t operator()(int i) const { return key[i]; }
+  __device__ __forceinline__ uint32_t operator()(int i, int) const { return key[i]; }
 };
 
 template <class Src, class D>
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(const BlockDesc* __rest
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     const int i = bd.begin + r * kThreads + (int)threadIdx.x;
-    d[r] = i < bd.end ? D::get(src(i)) : kRadix - 1;
+    d[r] = i < bd.end ? D::get(src(i, bd.event)) : kRadix - 1;
   }
   __syncthreads();
 #pragma unroll
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const BlockDesc* __r
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     const int i = first + r * kWave;
-    key[r] = i < bd.end ? src(i) : 0u;
+    key[r] = i < bd.end ? src(i, bd.event) : 0u;
   }
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {

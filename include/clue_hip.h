@@ -166,7 +166,13 @@ int32_t clue2d_run_host(clue_ctx* ctx, const clue2d_params* p, int32_t n, const 
                         int32_t* isSeed, int32_t* clusterIndex);
 
 /* A packed batch of events (event e = points offsets[e] .. offsets[e+1]-1; host offsets, device SoA),
- * asynchronous. No reference counterpart: the reference runs one event per produce call. */
+ * asynchronous on `stream`, as the reference's CUDA backend enqueues its copies and launches
+ * (src/cuda/plugin-CLUEClusterizer/CLUEAlgoCUDA.cu:31-38,53-79): the batch's block table is built on
+ * the host into one of 4 pinned buffers of the context and copied with hipMemcpyAsync on `stream`.
+ * The call waits for the device only when the context's workspace must grow (a larger batch than
+ * any before) or when the pinned buffer it takes still awaits its copy (4 earlier batch calls not
+ * yet started on the device). `offsets` may be reused as soon as the call returns. No reference
+ * counterpart otherwise: the reference runs one event per produce call. */
 int32_t clue2d_run_batch_device(clue_ctx* ctx, const clue2d_params* p, int32_t n_events, const int64_t* offsets,
                                 const float* x, const float* y, const int32_t* layer, const float* weight, float* rho,
                                 float* delta, int32_t* nearestHigher, int32_t* isSeed, int32_t* clusterIndex,
@@ -203,6 +209,7 @@ int32_t clue3d_run_host(clue_ctx* ctx, const clue3d_params* p, int32_t n, const 
                         int32_t* delta_second, int32_t* nh_layer, int32_t* nh_index, int32_t* isSeed,
                         int32_t* tracksterIndex);
 
+/* A packed CLUE3D batch, asynchronous in the same way as clue2d_run_batch_device. */
 int32_t clue3d_run_batch_device(clue_ctx* ctx, const clue3d_params* p, int32_t n_events, const int64_t* offsets,
                                 const float* z, const float* eta, const float* phi, const float* r_over_absz,
                                 const float* radius, const float* energy, const int32_t* layer,

@@ -54,7 +54,7 @@ int main(int argc, char** argv) {
   const float dc2 = sqrt_le(dc), dm2 = sqrt_le(odf * dc), dm2up = std::nextafter(dm2, INFINITY);
   if (nb > 0) {
     const int nHist = kRadix * nb;
-    Keys2D keys{in.x.data(), in.y.data(), in.layer.data(), k2.data(), &err};
+    Keys2D keys{in.x.data(), in.y.data(), in.layer.data(), k2.data(), &err, 0};
     emu::launch(nb, kThreads, [&] { k_radix_hist<Keys2D, DigitY>(blocks.data(), keys, hist.data()); });
     scan_exclusive(hist.data(), hist.data(), nHist, part.data());
     emu::launch(nb, kThreads, [&] {

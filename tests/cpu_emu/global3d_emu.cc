@@ -69,7 +69,7 @@ int main(int argc, char** argv) {
   int err = 0;
   if (nb > 0) {
     const int nHist = kRadix * nb;
-    Keys3D keys{eta.data(), phi.data(), layer.data(), k2.data(), &err};
+    Keys3D keys{eta.data(), phi.data(), layer.data(), k2.data(), &err, 0};
     emu::launch(nb, kThreads, [&] { k_radix_hist<Keys3D, DigitPhi>(blocks.data(), keys, hist.data()); });
     scan_exclusive(hist.data(), hist.data(), nHist, part.data());
     emu::launch(nb, kThreads, [&] {

@@ -211,6 +211,22 @@ def test_emulated_clue3d_params(emulator, tmp_path, oracle_lib, kw):
     helpers.assert_bit_exact(got, oracle_lib.clue3d(ev, p), helpers.OUT3D, f"emulated clue3d {kw}")
 
 
+def test_emulated_clue3d_bad_phi_is_flagged(emulator, tmp_path, oracle_lib):
+    """NaN and infinite phi have no tile bin: the run flags kErrBinRange, stays inside every buffer
+    (AddressSanitizer, LDS index checks) and leaves the other events of the batch exact."""
+    evs = [synth.clue3d_event(e, n_layers=20, mean_per_layer=60.0) for e in range(3)]
+    bad = {k: v.copy() for k, v in evs[1].items()}
+    bad["phi"][[0, 5, 17]] = np.array([np.nan, np.inf, -np.inf], np.float32)
+    packed, offs = synth.pack([evs[0], bad, evs[2]])
+    p = oracle.clue3d_params(S=2)
+    err, got = run_emulated3d(tmp_path, packed, offs, p)
+    assert err & 2
+    for e in (0, 2):
+        a, b = int(offs[e]), int(offs[e + 1])
+        helpers.assert_bit_exact({k: v[a:b] for k, v in got.items()}, oracle_lib.clue3d(evs[e], p), helpers.OUT3D,
+                                 f"event {e} beside a bad-phi event")
+
+
 EXE_SCAN = EMU / "_build" / "scan_emu"
 
 

@@ -43,3 +43,42 @@ def test_stream_callback_follows_the_enqueued_work(dev, oracle_lib):
     assert dev.lib.clue_stream_callback(dev.ctx, None, None, None) == -1
     for a in list(din.values()) + list(dout.values()):
         a.free()
+
+
+def _batch(dev, packed, offs, fields_in, fields_out):
+    N = int(offs[-1])
+    return dev.upload_soa(packed, fields_in), dev.alloc_soa(N, fields_out)
+
+
+def test_batches_enqueue_back_to_back_without_waiting(dev, oracle_lib):
+    """Two different CLUE2D batches and two CLUE3D batches are enqueued on one stream with no host
+    synchronisation between them (the second of each kind while the first still runs, smaller so
+    that the shared workspace does not grow); every event of all four matches the oracle. The block
+    tables travel through the context's pinned ring and stream-ordered copies, so the second batch's
+    tables cannot overwrite the first's before its kernels ran."""
+    s = dev.lib.clue_stream_create(dev.ctx)
+    assert s
+    b2 = [synth.pack([synth.clue2d_event(n, 70 + i, shuffle=(i % 2 == 1)) for i, n in enumerate(ns)])
+          for ns in ([900_000, 4_000, 250_000, 1_000], [20_000, 333_333, 7_777])]
+    b3 = [synth.clue3d_batch(k, first_event=40 * k) for k in (24, 9)]
+    io2 = [_batch(dev, p, o, clue_hip.FIELDS_2D_IN, clue_hip.FIELDS_2D_OUT) for p, o in b2]
+    io3 = [_batch(dev, p, o, clue_hip.FIELDS_3D_IN, clue_hip.FIELDS_3D_OUT) for p, o in b3]
+    p3 = clue_hip.params3d(S=2)
+    print("case back-to-back batches: 2D 1155000 + 361110 points, 3D 24 + 9 events", flush=True)
+    dev.sync()
+    for (din, dout), (_, offs) in zip(io2, b2):
+        dev.clue2d_batch(din, dout, offs, stream=s)
+    for (din, dout), (_, offs) in zip(io3, b3):
+        dev.clue3d_batch(din, dout, offs, p3, stream=s)
+    dev.check(s)
+    import oracle
+    for k, ((din, dout), (packed, offs)) in enumerate(zip(io2, b2)):
+        helpers.assert_bit_exact(dev.download_soa(dout), oracle_lib.clue2d_batch(packed, offs), helpers.OUT2D,
+                                 f"2d batch {k}")
+    for k, ((din, dout), (packed, offs)) in enumerate(zip(io3, b3)):
+        helpers.assert_bit_exact(dev.download_soa(dout), oracle_lib.clue3d_batch(packed, offs, oracle.clue3d_params(S=2)),
+                                 helpers.OUT3D, f"3d batch {k}")
+    for din, dout in io2 + io3:
+        for a in list(din.values()) + list(dout.values()):
+            a.free()
+    assert dev.lib.clue_stream_destroy(dev.ctx, s) == 0

@@ -86,3 +86,49 @@ def test_server_poisson_stream(dev):
     assert st["events"] == 300 and np.all(lat > 0)
     print(f"poisson: p50 {np.percentile(lat, 50):.3f} ms p99 {np.percentile(lat, 99):.3f} ms, "
           f"{st['batches']} batches", flush=True)
+
+
+def test_server_flags_a_bad_event_alone(dev, oracle_lib):
+    """A CLUE3D event whose phi has no tile bin (NaN, infinite) fails alone with CLUE_ERR_BIN; the
+    events batched with it are exact, and the context's own error flag stays clear."""
+    p3 = clue_hip.params3d(S=2)
+    evs = [synth.clue3d_event(960 + k) for k in range(5)]
+    bad = {k: v.copy() for k, v in evs[2].items()}
+    bad["phi"][[3, 40]] = np.array([np.nan, np.inf], np.float32)
+    s = clue_hip.Server(dev, p3=p3, max_points_2d=100_000, max_points_3d=200_000, max_events=16, slots=2)
+    print("case server bad-phi event in a batch", flush=True)
+    ids = [s.submit3d(bad if k == 2 else e) for k, e in enumerate(evs)]
+    for k, i in enumerate(ids):
+        if k == 2:
+            with pytest.raises(clue_hip.ClueError) as e:
+                s.wait(i)
+            assert e.value.code == -5
+        else:
+            out, _ = s.wait(i)
+            helpers.assert_bit_exact(out, oracle_lib.clue3d(evs[k], oracle.clue3d_params(S=2)), helpers.OUT3D,
+                                     f"event {k} beside the bad one")
+    s.close()
+    dev.check()
+
+
+def test_server_destroy_finishes_running_batches(dev, oracle_lib):
+    """Destroying a server with batches still queued and on the device finishes them first: every
+    caller buffer, pinned or pageable, holds the event's result afterwards."""
+    evs = [synth.clue2d_event(n, 980 + k) for k, n in enumerate([300_000, 2_000, 150_000, 40_000, 600_000, 9_000,
+                                                                  75_000, 1_000])]
+    s = clue_hip.Server(dev, max_points_2d=1_000_000, max_points_3d=1_000, max_events=2, slots=3)
+    print("case server destroy with batches in flight", flush=True)
+    outs = []
+    for k, e in enumerate(evs):
+        n = e["x"].size
+        if k % 2 == 0:  # pinned outputs: copied by DMA straight from the device
+            o = {f: dev.pinned_copy(np.full(n, -7, np.int32 if f in ("nearestHigher", "isSeed", "clusterIndex")
+                                            else np.float32)) for f in clue_hip.FIELDS_2D_OUT}
+        else:
+            o = None
+        rid = s.submit2d(e, o)
+        outs.append(s.outputs(rid))
+    s.close()
+    for k, (e, o) in enumerate(zip(evs, outs)):
+        helpers.assert_bit_exact(o, oracle_lib.clue2d(e["x"], e["y"], e["layer"], e["weight"]), helpers.OUT2D,
+                                 f"event {k} after destroy")
