@@ -37,12 +37,13 @@ BYTES_PER_CLUSTER_3D = 56
 TRAFFIC_PROFILE = REPO / "profiles" / "clue2d_batched_256ev_hbm_traffic.md"
 
 
-def measured_traffic():
-    """HBM bytes per graph replay of the 256-event batch, from the rocprofv3 FETCH_SIZE / WRITE_SIZE
-    summary (scaled by the copy-probe calibration recorded there), or None when absent."""
+def measured_traffic(key="traffic_bytes_per_launch"):
+    """HBM bytes per launch of the 256-event batch (the whole graph replay, or one kernel with
+    key = "traffic_bytes_<kernel>"), from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the
+    copy-probe calibration recorded there), or None when absent."""
     try:
         for line in TRAFFIC_PROFILE.read_text().splitlines():
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return int(line.split(":", 1)[1])
     except OSError:
         pass
@@ -182,6 +183,50 @@ def clue3d_leg(dev, n_events, steps, warmup):
     return out
 
 
+def shuffled_leg(dev, counts, p, steps, warmup):
+    """The same 256 events with their hits in random order (as raw input files need not be ordered by
+    layer): an autotuned graph plan timed like the headline step; four events checked against the
+    CPU oracle."""
+    packed, offs = synth.clue2d_batch(counts, shuffle=True)
+    N = int(offs[-1])
+    din = dev.upload_soa(packed, clue_hip.FIELDS_2D_IN)
+    dout = dev.alloc_soa(N, clue_hip.FIELDS_2D_OUT)
+    plan = dev.clue2d_plan(din, dout, offs, p, use_graph=True, autotune=True)
+    for _ in range(max(2, warmup)):
+        plan.launch()
+    dev.check()
+    lib, ctx = dev.lib, dev.ctx
+    dev.sync()
+    lib.clue_mark(ctx, 6, None)
+    for _ in range(steps):
+        plan.launch()
+    lib.clue_mark(ctx, 7, None)
+    dev.sync()
+    dev.check()
+    ms = lib.clue_elapsed_ms(ctx, 6, 7) / steps
+    out = dev.download_soa(dout)
+    sys.path.insert(0, str(REPO / "oracle"))
+    import oracle
+    O = oracle.Oracle()
+    checked = [0, 1, int(np.argmax(np.diff(offs))), len(counts) - 1]
+    same = True
+    for e in checked:
+        a, b = int(offs[e]), int(offs[e + 1])
+        want = O.clue2d(packed["x"][a:b], packed["y"][a:b], packed["layer"][a:b], packed["weight"][a:b],
+                        p.dc, p.rhoc, p.outlierDeltaFactor)
+        same = same and all(np.array_equal(np.ascontiguousarray(out[k][a:b]).view(np.int32),
+                                           np.ascontiguousarray(want[k]).view(np.int32)) for k in want)
+    tuning = plan.tuning
+    plan.destroy()
+    for arr in list(din.values()) + list(dout.values()):
+        arr.free()
+    value = N / (ms * 1e-3) / 1e6
+    log(f"shuffled hit order: {ms:.3f} ms/step ({value:.1f} Mhits/s), oracle events {checked} bit-exact={same}")
+    return {"workload": "clue2d_batched_256_events_loguniform_1e3_1e6_shuffled_hit_order", "hits": N,
+            "ms_per_step": round(ms, 4), "value": round(value, 2), "unit": "Mhits/s", "autotuned": tuning,
+            "events_checked_against_oracle": checked, "labels_bit_exact_vs_oracle": bool(same)}
+
+
 def pcie_leg(dev, packed, din, dout, plan, N, steps, batch_out):
     """The batched step with its host transfers: pinned host -> device copies of the inputs, the graph,
     device -> host copies of the outputs, all on the launch stream (PCIe-inclusive rate)."""
@@ -243,6 +288,9 @@ def c2_run(args):
     print(json.dumps(out), flush=True)
 
 
+SERVE_BUDGET = 64 << 30  # HBM budget the serving loop admits events against
+
+
 def serving_run(dev, pool, order, arrivals, slots, max_events, p3):
     """Open-loop Poisson stream: submit each event at its arrival time, wait for all."""
     srv = clue_hip.Server(dev, p3=p3, max_points_2d=4_000_000, max_points_3d=1_000_000, max_events=max_events,
@@ -257,14 +305,18 @@ def serving_run(dev, pool, order, arrivals, slots, max_events, p3):
     lat = np.array([srv.wait(i, 120_000)[1] for i in ids])
     span = time.perf_counter() - t0
     st = srv.stats()
+    st["device_bytes_in_use"] = int(dev.lib.clue_bytes_in_use(dev.ctx))
     srv.close()
     return lat, span, st
 
 
-def serving_leg(dev, rate, seconds, seed=11):  # noqa: C901
+def serving_leg(rate, seconds, seed=11):  # noqa: C901
     """C5: events arrive as a Poisson stream, 70 % CLUE2D (1e3-1e6 hits, log-uniform) and 30 % CLUE3D
     (50 layers x Poisson(200) layer clusters); the continuous-batching server (3 slots, up to 64 events
-    per batch) and per-event processing (1 slot, 1 event per batch) see the same stream."""
+    per batch) and per-event processing (1 slot, 1 event per batch) see the same stream. Each runs
+    on a context of its own with a 64 GB HBM budget, which its slots share and admit batches
+    against."""
+    dev = clue_hip.Device(0, budget_bytes=SERVE_BUDGET)
     rng = np.random.Generator(np.random.PCG64(seed))
     n2 = synth.c3_hit_counts(96, seed=seed)
     pool = [(2, synth.clue2d_event(int(n), 5000 + k)) for k, n in enumerate(n2)]
@@ -286,9 +338,11 @@ def serving_leg(dev, rate, seconds, seed=11):  # noqa: C901
         lat, span, st = serving_run(dev, pool, order, arrivals, slots, maxev, p3)
         out[name] = {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
                      "mean_ms": round(float(lat.mean()), 3), "mhits_per_s": round(hits / span / 1e6, 1),
-                     "batches": st["batches"], "slots": slots, "max_events_per_batch": maxev}
+                     "batches": st["batches"], "slots": slots, "max_events_per_batch": maxev,
+                     "hbm_budget_bytes": SERVE_BUDGET, "device_bytes_in_use": st["device_bytes_in_use"]}
         log(f"serving {name} at {rate}/s: p50 {out[name]['p50_ms']} ms p99 {out[name]['p99_ms']} ms, "
             f"{st['batches']} batches")
+    dev.close()
     return out
 
 
@@ -316,12 +370,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-per-event", action="store_true", help="skip the per-event comparison leg")
     ap.add_argument("--serve-rates", default="500,1500", help="serving leg: event rates (events/s), comma-separated")
-    ap.add_argument("--serve-seconds", type=float, default=2.0, help="serving leg: length of the stream")
+    ap.add_argument("--serve-seconds", type=float, default=4.0, help="serving leg: length of the stream")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-3d", action="store_true", help="skip the CLUE3D (512-event) leg")
-    ap.add_argument("--workload", choices=("c3", "c2"), default="c3",
-                    help="c3: the 256-event batch (default, the headline line); c2: only one 1e6-hit event")
+    ap.add_argument("--workload", choices=("c3", "c2", "c3s"), default="c3",
+                    help="c3: the 256-event batch (default, the headline line); c2: only one 1e6-hit event; "
+                         "c3s: the 256-event batch with its hits in random order as the timed step (profiling)")
     ap.add_argument("--no-pcie", action="store_true", help="skip the PCIe-inclusive figure")
+    ap.add_argument("--no-shuffled", action="store_true", help="skip the shuffled-hit-order leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run 3 device-to-device copies of BYTES (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -338,7 +394,7 @@ def main():
         return
 
     counts = synth.c3_hit_counts(args.events)
-    packed, offs = synth.clue2d_batch(counts)
+    packed, offs = synth.clue2d_batch(counts, shuffle=args.workload == "c3s")
     N = int(offs[-1])
     log(f"workload: {args.events} events, {N} hits")
     dev = clue_hip.Device(0)
@@ -376,19 +432,32 @@ def main():
                    "autotuned": tuning},
     }
     algo_bytes = BYTES_PER_HIT_2D * N
+    # the dominant kernel, re-run on the plan's stream between HIP events right after the timed steps
+    kms = {"k_density2d": plan.kernel_ms(0, reps=10), "k_nh2d": plan.kernel_ms(1, reps=10)}
+    kname = max(kms, key=kms.get)
+    kachieved = algo_bytes / (kms[kname] * 1e-3) / 1e9
+    ktraffic = measured_traffic(f"traffic_bytes_{kname}") if args.events == 256 else None
     achieved = algo_bytes / (ms * 1e-3) / 1e9
     traffic = measured_traffic() if args.events == 256 else None
-    result["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                          "kernel": "batched CLUE2D launch (all kernels of one graph replay)",
-                          "algorithmic_bytes_per_launch": algo_bytes}
+    result["roofline"] = {"bound": "hbm", "achieved": round(kachieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(kachieved / HBM_PEAK_GBS, 4), "traffic": ktraffic,
+                          "kernel": kname, "kernel_ms": round(kms[kname], 4),
+                          "kernel_ms_all": {k: round(v, 4) for k, v in kms.items()},
+                          "algorithmic_bytes_per_launch": algo_bytes,
+                          "replay": {"achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4),
+                                     "traffic": traffic,
+                                     "kernels": "all kernels of one graph replay of the batched path"}}
     if traffic:
         result["roofline"]["traffic_source"] = str(TRAFFIC_PROFILE.relative_to(REPO))
-        result["roofline"]["traffic_over_algorithmic"] = round(traffic / algo_bytes, 2)
-        result["roofline"]["hbm_gbs_measured_traffic"] = round(traffic / (ms * 1e-3) / 1e9, 1)
+        result["roofline"]["replay"]["traffic_over_algorithmic"] = round(traffic / algo_bytes, 2)
+        result["roofline"]["replay"]["hbm_gbs_measured_traffic"] = round(traffic / (ms * 1e-3) / 1e9, 1)
+    log(f"dominant kernel {kname}: {kms[kname]:.3f} ms ({kachieved:.0f} GB/s algorithmic), all {kms}")
 
     if not args.no_pcie:
         result["pcie_inclusive"] = pcie_leg(dev, packed, din, dout, plan, N, min(args.steps, 5), batch_out)
+
+    if not args.no_shuffled:
+        result["c3_shuffled"] = shuffled_leg(dev, counts, p, args.steps, args.warmup)
 
     if not args.no_per_event:
         pout = dev.alloc_soa(N, clue_hip.FIELDS_2D_OUT)
@@ -406,7 +475,7 @@ def main():
         result["clue3d"] = clue3d_leg(dev, 512, args.steps, args.warmup)
 
     if not args.no_serve:
-        result["serving"] = [serving_leg(dev, float(r), args.serve_seconds) for r in args.serve_rates.split(",")]
+        result["serving"] = [serving_leg(float(r), args.serve_seconds) for r in args.serve_rates.split(",")]
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(packed, offs, p, batch_out)

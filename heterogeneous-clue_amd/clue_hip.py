@@ -34,6 +34,7 @@ EXPORTS = [
     "clue_plan_workspace_bytes", "clue_selftest_arith", "clue_server_create", "clue_server_destroy",
     "clue_server_submit2d", "clue_server_submit3d", "clue_server_wait", "clue_server_stats",
     "clue_server_last_error", "clue_set_path", "clue_plan_path", "clue_plan_tuning", "clue2d_run_device_noalloc",
+    "clue_plan_kernel_ms",
 ]
 
 
@@ -119,6 +120,7 @@ def load_library(path: Path = LIB_PATH) -> C.CDLL:
         "clue_set_path": (I32, [VP, I32, I32]),
         "clue_plan_path": (I32, [VP]),
         "clue_plan_tuning": (I32, [VP, VP]),
+        "clue_plan_kernel_ms": (C.c_float, [VP, I32, I32, VP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -398,6 +400,13 @@ class Plan:
 
     def launch(self, stream=None):
         self.dev._chk(self.dev.lib.clue_plan_launch(self.h, stream))
+
+    def kernel_ms(self, which: int, reps: int = 10, stream=None) -> float:
+        """Mean ms of the plan's density (0) or nearest-higher (1) kernel, timed with HIP events."""
+        ms = float(self.dev.lib.clue_plan_kernel_ms(self.h, which, reps, stream))
+        if ms < 0:
+            raise ClueError(-1, "clue_plan_kernel_ms failed")
+        return ms
 
     @property
     def workspace_bytes(self) -> int:

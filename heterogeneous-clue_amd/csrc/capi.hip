@@ -952,6 +952,33 @@ int32_t clue_plan_tuning(const clue_plan* pl, int32_t* out4) {
 
 uint64_t clue_plan_workspace_bytes(const clue_plan* pl) { return pl ? pl->ws.bytes + pl->wsF.bytes : 0; }
 
+float clue_plan_kernel_ms(clue_plan* pl, int32_t which, int32_t reps, void* stream) {
+  if (!pl || which < 0 || which > 1 || reps < 1) return -1.f;
+  clue_ctx* c = pl->ctx;
+  const bool d3 = pl->dim == 3;
+  if (!d3 && pl->route2d != CLUE_PATH_GLOBAL) return -1.f;  // not launched yet (or not the global-sort path)
+  hipStream_t s = pick(c, stream);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return -1.f;
+  float ms = -1.f;
+  if (hipEventRecord(e0, s) == hipSuccess) {
+    for (int r = 0; r < reps; ++r) {
+      if (d3)
+        clue3d_enqueue_step(pl->b3, pl->io3, pl->p3, which, which ? pl->b3.nhVariant : pl->b3.densityVariant, s);
+      else
+        clue2d_enqueue_step(pl->b2, pl->io2, pl->p2, which, which ? pl->b2.nhVariant : pl->b2.densityVariant, s);
+    }
+    if (hipGetLastError() == hipSuccess && hipEventRecord(e1, s) == hipSuccess &&
+        hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+      ms /= (float)reps;
+    else
+      ms = -1.f;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms;
+}
+
 // ---------------------------------------------------------------- self-check
 
 int32_t clue_selftest_arith(clue_ctx* c, int32_t n, const float* v, const float* w, int32_t* xb, int32_t* yb,

@@ -236,6 +236,11 @@ int32_t clue_plan_path(const clue_plan* plan);
 int32_t clue_plan_tuning(const clue_plan* plan, int32_t* out4);
 void clue_plan_destroy(clue_plan* plan);
 uint64_t clue_plan_workspace_bytes(const clue_plan* plan);
+/* Measurement helper, no reference counterpart: mean milliseconds of one launch of the density
+ * (which = 0) or nearest-higher (which = 1) kernel of a plan that has been launched, with the
+ * variant it uses, re-run `reps` times on `stream` between two HIP events (both kernels only rewrite
+ * their outputs with the same values). Negative on error. */
+float clue_plan_kernel_ms(clue_plan* plan, int32_t which, int32_t reps, void* stream);
 
 /* ---------------------------------------------------------------- serving loop
  * No reference counterpart: the reference processes one event per produce call. A server owns

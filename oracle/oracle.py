@@ -198,4 +198,4 @@ def build(with_reference: bool | None = None) -> None:
     if with_reference is None:
         with_reference = ref_root.exists()
     if with_reference:
-        subprocess.run(["make", "-s", "-C", str(HERE), "ref", f"REF={ref_root}"], check=True)
+        subprocess.run(["make", "-s", "-C", str(HERE), "ref", "ref-native", f"REF={ref_root}"], check=True)
