@@ -1,48 +1,39 @@
-// Conditions-producer registry: DEFINE_FWK_EVENTSETUP_MODULE(T) registers T, constructed from a
-// configuration path (reference: src/serial/Framework/ESPluginFactory.h:14-58).
+// Conditions-producer registry. Interface of the reference (src/serial/Framework/ESPluginFactory.h:
+// 14-58): DEFINE_FWK_EVENTSETUP_MODULE(T) for a producer constructed from a configuration path,
+// edm::ESPluginFactory::create(name, path), and the polymorphic class template
+// edm::ESPluginFactory::impl::Maker<T> whose typeinfo indexes the plugin libraries. The registry
+// behind them is NamedRegistry.
 #ifndef CLUE_FW_ESPluginFactory_h
 #define CLUE_FW_ESPluginFactory_h
 
 #include <filesystem>
 #include <memory>
 #include <string>
-#include <unordered_map>
 
 #include "Framework/ESProducer.h"
+#include "Framework/NamedRegistry.h"
 
 namespace edm {
-  namespace ESPluginFactory {
-    namespace impl {
-      class MakerBase {
-      public:
-        virtual ~MakerBase() = default;
-        virtual std::unique_ptr<ESProducer> create(std::filesystem::path const& config) const = 0;
-      };
+  // one instance for every library of a job: defined in libclue_framework (PluginFactory.cc)
+  extern template class NamedRegistry<ESProducer, std::filesystem::path const&>;
 
+  namespace ESPluginFactory {
+    using ProducerRegistry = NamedRegistry<ESProducer, std::filesystem::path const&>;
+
+    namespace impl {
       template <typename T>
-      class Maker : public MakerBase {
+      class Maker final : public Factory<ESProducer, std::filesystem::path const&> {
       public:
-        std::unique_ptr<ESProducer> create(std::filesystem::path const& config) const override {
+        std::unique_ptr<ESProducer> make(std::filesystem::path const& config) const override {
           return std::make_unique<T>(config);
         }
       };
 
-      class Registry {
-      public:
-        void add(std::string const& name, std::unique_ptr<MakerBase> maker);
-        MakerBase const* get(std::string const& name) const;
-
-      private:
-        std::unordered_map<std::string, std::unique_ptr<MakerBase>> makers_;
-      };
-
-      Registry& getGlobalRegistry();
-
       template <typename T>
-      class Registrar {
-      public:
-        explicit Registrar(std::string const& name) { getGlobalRegistry().add(name, std::make_unique<Maker<T>>()); }
-      };
+      bool enrol(char const* name) {
+        ProducerRegistry::instance().enrol(name, std::make_unique<Maker<T>>());
+        return true;
+      }
     }  // namespace impl
 
     std::unique_ptr<ESProducer> create(std::string const& name, std::filesystem::path const& config);
@@ -51,7 +42,8 @@ namespace edm {
 
 #define EDM_ES_PLUGIN_SYM(x, y) EDM_ES_PLUGIN_SYM2(x, y)
 #define EDM_ES_PLUGIN_SYM2(x, y) x##y
-#define DEFINE_FWK_EVENTSETUP_MODULE(type) \
-  static edm::ESPluginFactory::impl::Registrar<type> EDM_ES_PLUGIN_SYM(maker, __LINE__)(#type);
+#define DEFINE_FWK_EVENTSETUP_MODULE(type)                                   \
+  static bool const EDM_ES_PLUGIN_SYM(edm_es_module_enrolled_, __LINE__) = \
+      edm::ESPluginFactory::impl::enrol<type>(#type);
 
 #endif
