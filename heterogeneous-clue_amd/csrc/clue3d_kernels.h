@@ -296,28 +296,77 @@ __device__ __forceinline__ void scan_window(const int* __restrict__ T, const Win
   }
 }
 
+// ---------------------------------------------------------------- staged candidates
+
+// The three fields a candidate step reads: {r_over_absz, phi, energy} for the density, {r_over_absz,
+// phi, rho} for the nearest higher (the fourth field of the tile-order records, z, is only read for
+// the query point, from registers). Staged as 12 bytes per candidate instead of 16, in two arrays so
+// that the two consecutive candidates of a walk step load with one 64-bit pair read and one 32-bit
+// pair read: xy[k] = {r_over_absz, phi}, z[k] = energy or rho.
+struct Cand3 {
+  float x, y, z;
+};
+struct Staged3 {
+  const float2* xy;
+  const float* z;
+  __device__ __forceinline__ Cand3 operator[](int q) const {
+    const float2 a = xy[q];
+    return Cand3{a.x, a.y, z[q]};
+  }
+};
+// Candidates read from the tile-order records in global memory (unstaged blocks).
+struct Global4 {
+  const float4* p;
+  __device__ __forceinline__ float4 operator[](int q) const { return p[q]; }
+};
+
+// Copy [lo, hi) (1 <= hi - lo <= STAGE) of the tile-order records into LDS, all loads of a thread in
+// flight together.
+template <int STAGE>
+__device__ __forceinline__ void stage_range3(const float4* __restrict__ g, float2* sxy, float* sz, int lo, int hi) {
+  static_assert(STAGE % kThreads == 0, "staging splits evenly over the block");
+  constexpr int kPer = STAGE / kThreads;
+  const int n = hi - lo;
+  float4 v[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int k = u * kThreads + (int)threadIdx.x;
+    v[u] = g[lo + (k < n ? k : n - 1)];
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int k = u * kThreads + (int)threadIdx.x;
+    if (k < n) {
+      sxy[k] = make_float2(v[u].x, v[u].y);
+      sz[k] = v[u].z;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- density
 
 // isReachable and the energy to add (CLUE3DAlgoKernels.h:39-43, :118-134) for candidates read
 // from `src` (the staged range or the whole sorted array). A contribution that is not reachable
 // adds +0.f, which leaves rho unchanged: rho starts at +0 and a round-to-nearest sum started from
 // +0 is never -0.
+template <class Src>
 struct DensityStep3D {
   int p;  // the point's own position (same frame as the candidates')
   int L, minL;
   float r0, zi, phii, dsq, kdf, kdfSame, kdfRun;
-  const float4* src;
+  Src src;
   float rho;
   __device__ __forceinline__ void run(int layerOff) { kdfRun = (minL + layerOff == L) ? kdfSame : kdf; }
-  __device__ __forceinline__ float term(const float4& c, float dp, int q) const {
+  template <class C>
+  __device__ __forceinline__ float term(const C& c, float dp, int q) const {
     const float r1 = c.x * zi;
     const float d = (r0 - r1) * (r0 - r1) + r1 * r1 * dp * dp;
     const float e = ((q == p) ? 1.f : kdfRun) * c.z;
     return d < dsq ? e : 0.f;
   }
   __device__ __forceinline__ void operator()(int q0, int q1, bool two) {
-    const float4 c0 = src[q0];
-    const float4 c1 = src[q1];
+    const auto c0 = src[q0];
+    const auto c1 = src[q1];
     float dp0 = phii - c0.y, dp1 = phii - c1.y;
     reduce_pair(dp0, dp1);
     const float e0 = term(c0, dp0, q0);
@@ -326,7 +375,7 @@ struct DensityStep3D {
     rho += two ? e1 : 0.f;
   }
   __device__ __forceinline__ void one(int q) {
-    const float4 c = src[q];
+    const auto c = src[q];
     rho += term(c, reduce_range(phii - c.y), q);
   }
 };
@@ -341,11 +390,15 @@ __global__ __launch_bounds__(kThreads) void k_density3d(
     float4* __restrict__ crp, float* __restrict__ rhoOut) {
   using Shape = CompShape3D<ITEMS>;
   static_assert(STAGE < (1 << kOffBits), "staged offsets fit an entry field");
-  __shared__ float4 s_st[STAGE];
+  __shared__ float2 s_xy[STAGE];
+  __shared__ float s_z[STAGE];
+  // the run lists; before them (block_range, then a barrier) its first words are the reduction scratch
   __shared__ uint32_t s_runs[kRunCap * kThreads];
-  __shared__ int s_red[2 * kWaves];
-  const BlockDesc bd = blocks[blockIdx.x / Shape::kPerSort];
-  const int b0 = bd.begin + (int)(blockIdx.x % Shape::kPerSort) * Shape::kHits;
+  int* s_red = reinterpret_cast<int*>(s_runs);
+  // consecutive compute blocks (overlapping staged ranges) run on one XCD and share its L2
+  const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  const BlockDesc bd = blocks[blk / Shape::kPerSort];
+  const int b0 = bd.begin + (blk % Shape::kPerSort) * Shape::kHits;
   const int b1 = (b0 + Shape::kHits) < bd.end ? (b0 + Shape::kHits) : bd.end;
   if (b0 >= b1) return;  // uniform over the block
   const int* T = table + (int64_t)bd.event * stride;
@@ -370,7 +423,7 @@ __global__ __launch_bounds__(kThreads) void k_density3d(
   int rlo, rhi;
   block_range(lo, hi, s_red, rlo, rhi);
   const bool staged = S <= kMaxListS && rhi - rlo <= STAGE;
-  if (staged && rhi > rlo) stage_range<STAGE>(cre, s_st, rlo, rhi);
+  if (staged && rhi > rlo) stage_range3<STAGE>(cre, s_xy, s_z, rlo, rhi);
   __syncthreads();
   const float kdf = prm.kernelDensityFactor;
   const float kdfSame = prm.kernelDensityFactor * (prm.densityOnSameLayer ? 1.f : 0.f);
@@ -381,16 +434,20 @@ __global__ __launch_bounds__(kThreads) void k_density3d(
       const int i = perm[p];
       const float zi = me[r].w;
       const float dsq = isSi[i] ? prm.densityXYDistanceSqr : radius[i] * radius[i];
-      DensityStep3D f{p, win[r].L, win[r].minL, me[r].x * zi, zi, me[r].y, dsq, kdf, kdfSame, kdf, cre, 0.f};
+      float rho;
       if (staged) {
-        f.p = p - rlo;
-        f.src = s_st;
+        DensityStep3D<Staged3> f{p - rlo, win[r].L, win[r].minL, me[r].x * zi, zi, me[r].y, dsq, kdf, kdfSame, kdf,
+                                 Staged3{s_xy, s_z}, 0.f};
         walk_window<2>(T, win[r], -1, rlo, s_runs + threadIdx.x, f);
+        rho = f.rho;
       } else {
+        DensityStep3D<Global4> f{p, win[r].L, win[r].minL, me[r].x * zi, zi, me[r].y, dsq, kdf, kdfSame, kdf,
+                                 Global4{cre}, 0.f};
         scan_window<2>(T, win[r], -1, f);
+        rho = f.rho;
       }
-      crp[p] = make_float4(me[r].x, me[r].y, f.rho, zi);
-      rhoOut[i] = f.rho;
+      crp[p] = make_float4(me[r].x, me[r].y, rho, zi);
+      rhoOut[i] = rho;
     }
   }
 }
@@ -399,13 +456,15 @@ __global__ __launch_bounds__(kThreads) void k_density3d(
 
 // distanceSqr and the "last minimum wins" update (CLUE3DAlgoKernels.h:296-299, :357-366) for
 // candidates read from `src`; bo keeps the sibling-layer offset of the current best.
+template <class Src>
 struct NhStep3D {
   float r0, zi, phii, rhoi;
-  const float4* src;
+  Src src;
   float best;
   int bq, bo, cur;
   __device__ __forceinline__ void run(int layerOff) { cur = layerOff; }
-  __device__ __forceinline__ void take(const float4& c, float dp, int q, bool valid) {
+  template <class C>
+  __device__ __forceinline__ void take(const C& c, float dp, int q, bool valid) {
     const float r1 = c.x * zi;
     const float d = (r0 - r1) * (r0 - r1) + r1 * r1 * dp * dp;
     // bitwise forms: lane masks combined without branches
@@ -415,15 +474,15 @@ struct NhStep3D {
     bo = upd ? cur : bo;
   }
   __device__ __forceinline__ void operator()(int q0, int q1, bool two) {
-    const float4 c0 = src[q0];
-    const float4 c1 = src[q1];
+    const auto c0 = src[q0];
+    const auto c1 = src[q1];
     float dp0 = phii - c0.y, dp1 = phii - c1.y;
     reduce_pair(dp0, dp1);
     take(c0, dp0, q0, true);
     take(c1, dp1, q1, two);
   }
   __device__ __forceinline__ void one(int q) {
-    const float4 c = src[q];
+    const auto c = src[q];
     take(c, reduce_range(phii - c.y), q, true);
   }
 };
@@ -441,11 +500,15 @@ __global__ __launch_bounds__(kThreads) void k_nh3d(
     int32_t* __restrict__ isSeedOut, int* __restrict__ parent) {
   using Shape = CompShape3D<ITEMS>;
   static_assert(STAGE < (1 << kOffBits), "staged offsets fit an entry field");
-  __shared__ float4 s_st[STAGE];
+  __shared__ float2 s_xy[STAGE];
+  __shared__ float s_z[STAGE];
+  // the run lists; before them (block_range, then a barrier) its first words are the reduction scratch
   __shared__ uint32_t s_runs[kRunCap * kThreads];
-  __shared__ int s_red[2 * kWaves];
-  const BlockDesc bd = blocks[blockIdx.x / Shape::kPerSort];
-  const int b0 = bd.begin + (int)(blockIdx.x % Shape::kPerSort) * Shape::kHits;
+  int* s_red = reinterpret_cast<int*>(s_runs);
+  // consecutive compute blocks (overlapping staged ranges) run on one XCD and share its L2
+  const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  const BlockDesc bd = blocks[blk / Shape::kPerSort];
+  const int b0 = bd.begin + (blk % Shape::kPerSort) * Shape::kHits;
   const int b1 = (b0 + Shape::kHits) < bd.end ? (b0 + Shape::kHits) : bd.end;
   if (b0 >= b1) return;
   const int* T = table + (int64_t)bd.event * stride;
@@ -471,7 +534,7 @@ __global__ __launch_bounds__(kThreads) void k_nh3d(
   int rlo, rhi;
   block_range(lo, hi, s_red, rlo, rhi);
   const bool staged = S <= kMaxListS && rhi - rlo <= STAGE;
-  if (staged && rhi > rlo) stage_range<STAGE>(crp, s_st, rlo, rhi);
+  if (staged && rhi > rlo) stage_range3<STAGE>(crp, s_xy, s_z, rlo, rhi);
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
@@ -482,21 +545,28 @@ __global__ __launch_bounds__(kThreads) void k_nh3d(
       const float zi = me[r].w;
       const float rhoi = me[r].z;
       const int skipL = prm.nearestHigherOnSameLayer ? -1 : L;
-      NhStep3D f{me[r].x * zi, zi, me[r].y, rhoi, crp, FLT_MAX, -1, 0, 0};
+      float best;
+      int bq, bo;
       if (staged) {
-        f.src = s_st;
+        NhStep3D<Staged3> f{me[r].x * zi, zi, me[r].y, rhoi, Staged3{s_xy, s_z}, FLT_MAX, -1, 0, 0};
         walk_window<1>(T, win[r], skipL, rlo, s_runs + threadIdx.x, f);
-        f.bq += rlo;
+        best = f.best;
+        bq = f.bq + rlo;
+        bo = f.bo;
       } else {
+        NhStep3D<Global4> f{me[r].x * zi, zi, me[r].y, rhoi, Global4{crp}, FLT_MAX, -1, 0, 0};
         scan_window<1>(T, win[r], skipL, f);
+        best = f.best;
+        bq = f.bq;
+        bo = f.bo;
       }
       float df;
       int dl, hl, hiG;
-      if (f.best != FLT_MAX) {
-        df = sqrtf(f.best);
-        hl = win[r].minL + f.bo;
+      if (best != FLT_MAX) {
+        df = sqrtf(best);
+        hl = win[r].minL + bo;
         dl = hl > L ? hl - L : L - hl;
-        hiG = perm[f.bq];
+        hiG = perm[bq];
       } else {
         df = FLT_MAX;
         dl = INT_MAX;
