@@ -373,9 +373,10 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="serving leg: length of the stream")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-3d", action="store_true", help="skip the CLUE3D (512-event) leg")
-    ap.add_argument("--workload", choices=("c3", "c2", "c3s"), default="c3",
+    ap.add_argument("--workload", choices=("c3", "c2", "c3s", "c4"), default="c3",
                     help="c3: the 256-event batch (default, the headline line); c2: only one 1e6-hit event; "
-                         "c3s: the 256-event batch with its hits in random order as the timed step (profiling)")
+                         "c3s: the 256-event batch with its hits in random order as the timed step (profiling); "
+                         "c4: only the 512-event CLUE3D batch (profiling)")
     ap.add_argument("--no-pcie", action="store_true", help="skip the PCIe-inclusive figure")
     ap.add_argument("--no-shuffled", action="store_true", help="skip the shuffled-hit-order leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -391,6 +392,11 @@ def main():
         return
     if args.workload == "c2":
         c2_run(args)
+        return
+    if args.workload == "c4":
+        dev = clue_hip.Device(0)
+        print(json.dumps(clue3d_leg(dev, 512, args.steps, args.warmup)), flush=True)
+        dev.close()
         return
 
     counts = synth.c3_hit_counts(args.events)

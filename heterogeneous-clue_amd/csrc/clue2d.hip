@@ -106,7 +106,9 @@ void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DPara
       b.blocks, KeyArray{b.k2}, b.hist,
       EmitLast2D{b.i2, b.k1, io.x, io.y, io.weight, b.xyw});
   const uint32_t* keyS = b.k1;
-  build_table(keyS, b.blocks, b.nBlocks, b.evOff, b.E, k2dKeys, b.tableStride, b.table, b.tileStart, s);
+  // every lookup of a search box stays in the point's own layer (entries up to the layer's end)
+  build_table(keyS, b.blocks, b.nBlocks, b.evOff, b.E, k2dKeys, k2dBinsPerLayer, 0, b.tableStride, b.table,
+              b.tileStart, s);
   const int nTiles = b.nBlocks * kTilesPerSort;
   k_reach2d<<<(nTiles + kReachThreads - 1) / kReachThreads, kReachThreads, 0, s>>>(
       b.blocks, nTiles, keyS, b.table, b.tableStride, prm.dc, prm.outlierDeltaFactor, b.reachD, b.reachN);

@@ -71,7 +71,10 @@ void clue3d_enqueue(const Clue3DBuffers& b, const Clue3DIO& io, const Clue3DPara
       b.blocks, KeyArray{b.k2}, b.hist,
       EmitLast3D{b.i2, b.k1, b.i1, io.r_over_absz, io.phi, io.energy, io.z, b.cre});
   // sorted keys in k1, permutation in i1
-  build_table(b.k1, b.blocks, b.nBlocks, b.evOff, b.E, k3dKeys, b.tableStride, b.table, b.tileStart, s);
+  // a window reaches S sibling layers past a point's layer, and its eta row 34 the layer after those
+  const int S = prm.densitySiblingLayers > 0 ? prm.densitySiblingLayers : 0;
+  build_table(b.k1, b.blocks, b.nBlocks, b.evOff, b.E, k3dKeys, k3dBinsPerLayer, (S < k3dLayers ? S : k3dLayers) + 1,
+              b.tableStride, b.table, b.tileStart, s);
   density_step(b, io, prm, b.densityVariant, s);
   nh_step(b, io, prm, b.nhVariant, s);
   int* parent = reinterpret_cast<int*>(b.k2);

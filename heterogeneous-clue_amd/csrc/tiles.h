@@ -229,8 +229,11 @@ struct EmitMid {
 // bins starts (tileStart, nEvents * (table_tiles(nKeys) + 1) ints); then one block per
 // (event, tile) fills the tile's bins from the keys of that range.
 inline int table_tiles(int nKeys) { return (nKeys + 1 + kTableTile - 1) / kTableTile; }
+// Entries of an event beyond layerReach layers past its last occupied layer (binsPerLayer bins each) are
+// not written: callers guarantee that no lookup reaches them.
 void build_table(const uint32_t* keySorted, const BlockDesc* blocksDev, int nBlocks, const int* evOffDev,
-                 int nEvents, int nKeys, int64_t stride, int* table, int* tileStart, hipStream_t s);
+                 int nEvents, int nKeys, int binsPerLayer, int layerReach, int64_t stride, int* table,
+                 int* tileStart, hipStream_t s);
 
 // Sequential carving of one allocation into 256-byte aligned sub-buffers.
 struct Carver {

@@ -37,11 +37,13 @@ void scan_exclusive(const int* in, int* out, int64_t n, int* part, hipStream_t s
 }
 
 void build_table(const uint32_t* keySorted, const BlockDesc* blocksDev, int nBlocks, const int* evOffDev,
-                 int nEvents, int nKeys, int64_t stride, int* table, int* tileStart, hipStream_t s) {
+                 int nEvents, int nKeys, int binsPerLayer, int layerReach, int64_t stride, int* table,
+                 int* tileStart, hipStream_t s) {
   if (nEvents <= 0) return;
   const int tiles = table_tiles(nKeys);
   if (nBlocks > 0) k_tile_starts<<<nBlocks, kThreads, 0, s>>>(blocksDev, keySorted, evOffDev, tiles, tileStart);
-  k_table<<<nEvents * tiles, kThreads, 0, s>>>(keySorted, evOffDev, tileStart, tiles, nKeys, stride, table);
+  k_table<<<nEvents * tiles, kThreads, 0, s>>>(keySorted, evOffDev, tileStart, tiles, nKeys, binsPerLayer, layerReach,
+                                               stride, table);
 }
 
 void launch_labels(const BlockDesc* blocksDev, int nBlocks, int* parent, const int* isSeed, int64_t N,

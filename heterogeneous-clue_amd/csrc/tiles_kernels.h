@@ -118,9 +118,13 @@ __global__ __launch_bounds__(kThreads) void k_tile_starts(const BlockDesc* __res
 // [16t, 16t+16) fall into distinct banks.
 __device__ __forceinline__ int tpad(int j) { return j + j / kTablePer; }
 
+// Only the entries [0, lim) of an event are written, lim = (the event's last key's layer + layerReach +
+// 1) * binsPerLayer + 1 (at most nKeys + 1): no search window of a point reaches further, so the
+// table of an event that occupies few layers is written only as far as it is read.
 __global__ __launch_bounds__(kThreads) void k_table(const uint32_t* __restrict__ key, const int* __restrict__ evOff,
                                                     const int* __restrict__ tileStart, int tiles, int nKeys,
-                                                    int64_t stride, int* __restrict__ table) {
+                                                    int binsPerLayer, int layerReach, int64_t stride,
+                                                    int* __restrict__ table) {
   __shared__ int s_first[kTableTile + kTableTile / kTablePer];
   __shared__ int s_tmp[kWaves];
   const int e = blockIdx.x / tiles;
@@ -128,12 +132,13 @@ __global__ __launch_bounds__(kThreads) void k_table(const uint32_t* __restrict__
   const int b0 = tile * kTableTile;
   const int b1 = (b0 + kTableTile) < (nKeys + 1) ? (b0 + kTableTile) : (nKeys + 1);
   const int s = evOff[e], t = evOff[e + 1];
-  int lo = s, hi = s;
-  if (t > s) {  // tile starts are only recorded for non-empty events
-    const int* TS = tileStart + (int64_t)e * (tiles + 1);
-    lo = TS[tile];
-    hi = TS[tile + 1];
-  }
+  if (t <= s) return;  // no point of an empty event looks up the table (uniform over the block)
+  const int64_t limL = ((int64_t)(key[t - 1] / (uint32_t)binsPerLayer) + layerReach + 1) * binsPerLayer + 1;
+  const int lim = limL < (int64_t)nKeys + 1 ? (int)limL : nKeys + 1;
+  if (b0 >= lim) return;  // uniform over the block
+  const int* TS = tileStart + (int64_t)e * (tiles + 1);
+  const int lo = TS[tile], hi = TS[tile + 1];
+  const int bEnd = b1 < lim ? b1 : lim;
   for (int j = threadIdx.x; j < kTableTile; j += kThreads) s_first[tpad(j)] = 0x7fffffff;
   __syncthreads();
   // first position of every non-empty bin of the tile (one writer per bin: the run start); four
@@ -173,7 +178,7 @@ __global__ __launch_bounds__(kThreads) void k_table(const uint32_t* __restrict__
   }
   __syncthreads();
   int* out = table + (int64_t)e * stride + b0;
-  for (int j = threadIdx.x; j < b1 - b0; j += kThreads) __builtin_nontemporal_store(s_first[tpad(j)], &out[j]);
+  for (int j = threadIdx.x; j < bEnd - b0; j += kThreads) __builtin_nontemporal_store(s_first[tpad(j)], &out[j]);
 }
 
 

@@ -74,7 +74,8 @@ int main(int argc, char** argv) {
           blocks.data(), KeyArray{k2.data()}, hist.data(),
           EmitLast2D{i2.data(), k1.data(), in.x.data(), in.y.data(), in.w.data(), xyw.data()});
     });
-    build_table(k1.data(), blocks.data(), nb, evOff.data(), E, k2dKeys, stride, table.data(), tileStart.data());
+    build_table(k1.data(), blocks.data(), nb, evOff.data(), E, k2dKeys, k2dBinsPerLayer, 0, stride, table.data(),
+                tileStart.data());
     const int nTiles = nb * kTilesPerSort;
     std::vector<int4> reachD(nTiles, int4{P, P, P, P}), reachN(nTiles, int4{P, P, P, P});
     emu::launch((nTiles + kReachThreads - 1) / kReachThreads, kReachThreads, [&] {

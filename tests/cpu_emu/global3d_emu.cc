@@ -89,7 +89,11 @@ int main(int argc, char** argv) {
           blocks.data(), KeyArray{k2.data()}, hist.data(),
           EmitLast3D{i2.data(), k1.data(), i1.data(), roz.data(), phi.data(), energy.data(), z.data(), cre.data()});
     });
-    build_table(k1.data(), blocks.data(), nb, evOff.data(), E, k3dKeys, stride, table.data(), tileStart.data());
+    {
+      const int S = prm.densitySiblingLayers > 0 ? prm.densitySiblingLayers : 0;
+      build_table(k1.data(), blocks.data(), nb, evOff.data(), E, k3dKeys, k3dBinsPerLayer,
+                  (S < k3dLayers ? S : k3dLayers) + 1, stride, table.data(), tileStart.data());
+    }
     const int dv = argc > 3 ? atoi(argv[3]) : 0, nv = argc > 4 ? atoi(argv[4]) : 0;
     dispatch(dv, [&](auto I, auto S) {
       emu::launch(CompShape3D<I>::kPerSort * nb, kThreads, [&] {

@@ -29,11 +29,12 @@ inline void scan_exclusive(const int* in, int* out, int64_t n, int* part, int di
 }
 
 inline void build_table(const uint32_t* key, const BlockDesc* blocks, int nBlocks, const int* evOff, int nEvents,
-                        int nKeys, int64_t stride, int* table, int* tileStart) {
+                        int nKeys, int binsPerLayer, int layerReach, int64_t stride, int* table, int* tileStart) {
   if (nEvents <= 0) return;
   const int tiles = table_tiles(nKeys);
   if (nBlocks > 0) emu::launch(nBlocks, kThreads, [&] { k_tile_starts(blocks, key, evOff, tiles, tileStart); });
-  emu::launch(nEvents * tiles, kThreads, [&] { k_table(key, evOff, tileStart, tiles, nKeys, stride, table); });
+  emu::launch(nEvents * tiles, kThreads,
+              [&] { k_table(key, evOff, tileStart, tiles, nKeys, binsPerLayer, layerReach, stride, table); });
 }
 
 inline void launch_labels(const BlockDesc* blocks, int nBlocks, int* parent, const int* isSeed, int64_t N,
