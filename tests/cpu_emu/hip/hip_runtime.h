@@ -23,12 +23,20 @@
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+#ifdef EMU_LDS_REDZONES
+// Every `__shared__` array a plain static: AddressSanitizer puts a redzone after each one, so an
+// index past the end of an LDS array stops the run here (on the device such an access is dropped
+// or reads another array, silently). LDS is not poisoned in this build.
+#define __shared__ static
+#else
 // Every `__shared__` array lives in one linker section, which launch() fills with a poison byte
 // before each workgroup: LDS starts with arbitrary contents on the device, so code that reads LDS
 // before writing it must not pass here by reading zeros or a previous workgroup's values.
+// (AddressSanitizer leaves globals of a named section without redzones.)
 #define __shared__ static __attribute__((section("emu_lds")))
 extern "C" char __start_emu_lds[];
 extern "C" char __stop_emu_lds[];
+#endif
 #define __align__(n) __attribute__((aligned(n)))
 #define __builtin_amdgcn_sched_barrier(x) ((void)0)
 
@@ -129,7 +137,9 @@ void launch(unsigned grid, unsigned block, F&& body) {
       g.waves[w].bar = std::make_unique<std::barrier<>>(lanes);
     }
     group = &g;
+#ifndef EMU_LDS_REDZONES
     std::memset(__start_emu_lds, lds_poison(), (size_t)(__stop_emu_lds - __start_emu_lds));
+#endif
     start.arrive_and_wait();
     done.arrive_and_wait();
     group = nullptr;

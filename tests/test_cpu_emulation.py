@@ -10,6 +10,7 @@ bit-exact against the golden fixtures and the oracle:
 * the fused segment path (layer bucket pass, segment lists, fused segment kernels, seed numbering),
   which the library keeps disabled on the device (clue_set_path refuses it) until it has also
   passed on the GPU."""
+import os
 import subprocess
 from pathlib import Path
 
@@ -24,6 +25,8 @@ EMU = Path(__file__).resolve().parent / "cpu_emu"
 EXE = EMU / "_build" / "fused2d_emu"
 EXE_GLOBAL = EMU / "_build" / "global2d_emu"
 EXE_3D = EMU / "_build" / "global3d_emu"
+# the ThreadSanitizer builds report every race except the one k_labels intends (tsan.supp)
+ENV = {**os.environ, "TSAN_OPTIONS": f"suppressions={EMU / 'tsan.supp'} history_size=4"}
 
 
 @pytest.fixture(scope="module")
@@ -42,8 +45,9 @@ def run_emulated(exe, tmp_path, ev, offs, dc=3.0, rhoc=10.0, odf=2.0, wgs=2, ext
         f.write(np.asarray(offs, np.int64).tobytes())
         for k, dt in (("x", np.float32), ("y", np.float32), ("layer", np.int32), ("weight", np.float32)):
             f.write(np.ascontiguousarray(ev[k], dt).tobytes())
-    args = [str(exe), str(src), str(dst)] + ([str(wgs)] if exe == EXE else [str(a) for a in extra])
-    r = subprocess.run(args, capture_output=True, text=True, timeout=600)
+    fused = Path(exe).name.startswith("fused2d")
+    args = [str(exe), str(src), str(dst)] + ([str(wgs)] if fused else [str(a) for a in extra])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=600, env=ENV)
     assert r.returncode == 0, r.stderr[-4000:]
     d = np.fromfile(dst, np.int32)
     o = d[2:].reshape(5, N)
@@ -62,8 +66,8 @@ def test_emulated_fused_golden(emulator, tmp_path, name):
     helpers.assert_bit_exact(got, want, helpers.OUT2D, f"emulated {name}")
 
 
-def test_emulated_fused_batch_and_classes(emulator, tmp_path, oracle_lib):
-    """Several events (one empty) whose layers fall in all three size classes and on their edges."""
+def class_batch():
+    """Several events (one empty) whose layers fall in all three fused size classes and on their edges."""
     rng = np.random.Generator(np.random.PCG64(41))
     evs = []
     for n_layers, per in ((3, [1, 2, 1023]), (3, [1024, 1025, 7]), (2, [4096, 4097]), (0, [])):
@@ -81,7 +85,12 @@ def test_emulated_fused_batch_and_classes(emulator, tmp_path, oracle_lib):
         evs.append({"x": x[perm], "y": np.concatenate(ys).astype(np.float32)[perm],
                     "layer": np.concatenate(ls).astype(np.int32)[perm],
                     "weight": rng.exponential(1.0, x.size).astype(np.float32)})
-    packed, offs = synth.pack(evs)
+    return synth.pack(evs)
+
+
+def test_emulated_fused_batch_and_classes(emulator, tmp_path, oracle_lib):
+    """Several events (one empty) whose layers fall in all three size classes and on their edges."""
+    packed, offs = class_batch()
     err, big, got = run_emulated(emulator, tmp_path, packed, offs)
     assert err == 0 and big == 4097
     want = oracle_lib.clue2d_batch(packed, offs)
@@ -145,7 +154,7 @@ def test_emulated_global_extremes(emulator, tmp_path, oracle_lib, params):
 FIELDS_3D_F = ("z", "eta", "phi", "r_over_absz", "radius", "energy")
 
 
-def run_emulated3d(tmp_path, ev, offs, params, variants=()):
+def run_emulated3d(tmp_path, ev, offs, params, variants=(), exe=EXE_3D):
     N = int(offs[-1])
     src, dst = tmp_path / "in3.bin", tmp_path / "out3.bin"
     with open(src, "wb") as f:
@@ -156,8 +165,8 @@ def run_emulated3d(tmp_path, ev, offs, params, variants=()):
             f.write(np.ascontiguousarray(ev[k], np.float32).tobytes())
         for k in ("layer", "isSilicon"):
             f.write(np.ascontiguousarray(ev[k], np.int32).tobytes())
-    r = subprocess.run([str(EXE_3D), str(src), str(dst)] + [str(v) for v in variants], capture_output=True,
-                       text=True, timeout=900)
+    r = subprocess.run([str(exe), str(src), str(dst)] + [str(v) for v in variants], capture_output=True,
+                       text=True, timeout=900, env=ENV)
     assert r.returncode == 0, r.stderr[-4000:]
     d = np.fromfile(dst, np.int32)
     o = d[1:].reshape(7, N)
@@ -225,6 +234,65 @@ def test_emulated_clue3d_bad_phi_is_flagged(emulator, tmp_path, oracle_lib):
         a, b = int(offs[e]), int(offs[e + 1])
         helpers.assert_bit_exact({k: v[a:b] for k, v in got.items()}, oracle_lib.clue3d(evs[e], p), helpers.OUT3D,
                                  f"event {e} beside a bad-phi event")
+
+
+# ------------------------------------------------------------------ sanitizer builds
+
+SANITIZED = ("ldsz", "tsan")  # LDS arrays with redzones (AddressSanitizer); ThreadSanitizer
+
+
+def sanitized(name, variant):
+    return EMU / "_build" / f"{name}_{variant}"
+
+
+@pytest.mark.parametrize("variant", SANITIZED)
+def test_sanitized_fused(emulator, tmp_path, oracle_lib, variant):
+    """The fused segment kernels with every LDS array bounds-checked, and under ThreadSanitizer (a
+    barrier missing between two lanes' accesses to one LDS word is a reported race), on the size-class
+    batch and on a single event of four 1500-point layers, each run with two workgroups per class."""
+    packed, offs = class_batch()
+    err, big, got = run_emulated(sanitized("fused2d_emu", variant), tmp_path, packed, offs)
+    assert err == 0 and big == 4097
+    helpers.assert_bit_exact(got, oracle_lib.clue2d_batch(packed, offs), helpers.OUT2D, f"fused {variant}")
+    rng = np.random.Generator(np.random.PCG64(5))
+    n = 6000
+    ev = {"x": rng.uniform(-250, 250, n).astype(np.float32), "y": rng.uniform(-250, 250, n).astype(np.float32),
+          "layer": np.repeat(np.arange(4, dtype=np.int32), n // 4)[rng.permutation(n)],
+          "weight": rng.exponential(1.0, n).astype(np.float32)}
+    err, big, got = run_emulated(sanitized("fused2d_emu", variant), tmp_path, ev, np.array([0, n]))
+    assert err == 0 and big == 1500
+    helpers.assert_bit_exact(got, oracle_lib.clue2d(ev["x"], ev["y"], ev["layer"], ev["weight"]), helpers.OUT2D,
+                             f"fused 4 x 1500 {variant}")
+
+
+@pytest.mark.parametrize("variant", SANITIZED)
+def test_sanitized_global2d(emulator, tmp_path, oracle_lib, variant):
+    """The global-sort path's kernels, LDS bounds-checked and under ThreadSanitizer: a three-event batch and
+    the extreme-coordinate event with a very large distance."""
+    evs = [synth.clue2d_event(n, 80 + i) for i, n in enumerate([2500, 0, 700])]
+    packed, offs = synth.pack(evs)
+    err, _, got = run_emulated(sanitized("global2d_emu", variant), tmp_path, packed, offs)
+    assert err == 0
+    helpers.assert_bit_exact(got, oracle_lib.clue2d_batch(packed, offs), helpers.OUT2D, f"global {variant}")
+    ev = helpers.extreme_event(n_normal=200)
+    dc, rhoc, odf = helpers.EXTREME_PARAMS[5]
+    err, _, got = run_emulated(sanitized("global2d_emu", variant), tmp_path, ev, np.array([0, ev["x"].size]),
+                               dc, rhoc, odf)
+    assert err == 0
+    want = oracle_lib.clue2d(ev["x"], ev["y"], ev["layer"], ev["weight"], dc, rhoc, odf)
+    helpers.assert_bit_exact(got, want, helpers.OUT2D, f"global extremes {variant}")
+
+
+@pytest.mark.parametrize("variant", SANITIZED)
+def test_sanitized_clue3d(emulator, tmp_path, oracle_lib, variant):
+    """The CLUE3D kernels, LDS bounds-checked and under ThreadSanitizer, on a two-event batch (density over
+    +/-2 sibling layers) and with a reach beyond the run list's layer field."""
+    evs = [synth.clue3d_event(e, n_layers=30, mean_per_layer=80.0) for e in range(2)]
+    packed, offs = synth.pack(evs)
+    for p in (oracle.clue3d_params(S=2), oracle.clue3d_params(S=40)):
+        err, got = run_emulated3d(tmp_path, packed, offs, p, exe=sanitized("global3d_emu", variant))
+        assert err == 0
+        helpers.assert_bit_exact(got, oracle_lib.clue3d_batch(packed, offs, p), helpers.OUT3D, f"clue3d {variant}")
 
 
 EXE_SCAN = EMU / "_build" / "scan_emu"
