@@ -31,10 +31,10 @@ struct Clue2DBuffers {
   int nBlocks = 0;
   const BlockDesc* blocks = nullptr;  // device
   const int* evOff = nullptr;         // device, E + 1 (int32 offsets)
-  uint32_t* k1 = nullptr;
-  int* i1 = nullptr;
-  uint32_t* k2 = nullptr;
-  int* i2 = nullptr;
+  uint32_t* keyS = nullptr;  // tile order: bin keys
+  int* scr = nullptr;        // slots of layer segments too large for LDS (k_segsort2d)
+  float4* lg = nullptr;      // layer order: {x, y, weight, point index as int bits}; later the parent array
+  int* segB = nullptr;       // per event, the first position of each layer (E * (k2dLayers + 1))
   float4* xyw = nullptr;   // tile order: {x, y, weight, point index as int bits}
   uint32_t* rhoKey = nullptr;  // tile order: rho as an order-preserving unsigned key (rho_key)
   int4* reachD = nullptr;  // per 256-point tile: staging / table reach of the density boxes
@@ -42,7 +42,6 @@ struct Clue2DBuffers {
   int* hist = nullptr;
   int* part = nullptr;
   int* table = nullptr;
-  int* tileStart = nullptr;
   int64_t tableStride = 0;
   SeedRank seeds{};
   int* err = nullptr;  // device error flags: err[event * errStride]

@@ -88,33 +88,28 @@ static_assert(kNumVariants2D == 4, "launch_density2d / launch_nh2d dispatch four
 void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DParams& prm, hipStream_t s) {
   if (b.N <= 0 || b.nBlocks <= 0) return;
   const int nHist = kRadix * b.nBlocks;
-  // pass 1: yBin (keys computed and recorded in k2)
-  Keys2D keys{io.x, io.y, io.layer, b.k2, b.err, b.errStride};
-  k_radix_hist<Keys2D, DigitY><<<b.nBlocks, kThreads, 0, s>>>(b.blocks, keys, b.hist);
+  // tile order: a stable pass by layer into {x, y, w, index} records, then each (event, layer) sorted
+  // by bin in its own workgroup, which also writes that layer's bin-table entries
+  const LayerOf2D layerOf{io.layer, b.err, b.errStride};
+  k_radix_hist<LayerOf2D, DigitLayer2D><<<b.nBlocks, kThreads, 0, s>>>(b.blocks, layerOf, b.hist);
   scan_exclusive(b.hist, b.hist, nHist, b.part, s);
-  k_radix_scatter<KeyArray, DigitY, EmitFirst>
-      <<<b.nBlocks, kThreads, 0, s>>>(b.blocks, KeyArray{b.k2}, b.hist, EmitFirst{b.k1, b.i1});
-  // pass 2: xBin
-  k_radix_hist<KeyArray, DigitX><<<b.nBlocks, kThreads, 0, s>>>(b.blocks, KeyArray{b.k1}, b.hist);
-  scan_exclusive(b.hist, b.hist, nHist, b.part, s);
-  k_radix_scatter<KeyArray, DigitX, EmitMid>
-      <<<b.nBlocks, kThreads, 0, s>>>(b.blocks, KeyArray{b.k1}, b.hist, EmitMid{b.i1, b.k2, b.i2});
-  // pass 3: layer; gathers the SoA into tile order
-  k_radix_hist<KeyArray, DigitL><<<b.nBlocks, kThreads, 0, s>>>(b.blocks, KeyArray{b.k2}, b.hist);
-  scan_exclusive(b.hist, b.hist, nHist, b.part, s);
-  k_radix_scatter<KeyArray, DigitL, EmitLast2D><<<b.nBlocks, kThreads, 0, s>>>(
-      b.blocks, KeyArray{b.k2}, b.hist,
-      EmitLast2D{b.i2, b.k1, io.x, io.y, io.weight, b.xyw});
-  const uint32_t* keyS = b.k1;
-  // every lookup of a search box stays in the point's own layer (entries up to the layer's end)
-  build_table(keyS, b.blocks, b.nBlocks, b.evOff, b.E, k2dKeys, k2dBinsPerLayer, 0, b.tableStride, b.table,
-              b.tileStart, s);
+  k_radix_scatter<LayerOf2D, DigitLayer2D, EmitSeg2D><<<b.nBlocks, kThreads, 0, s>>>(
+      b.blocks, layerOf, b.hist, EmitSeg2D{io.x, io.y, io.weight, b.lg});
+  k_layer_starts2d<<<b.nBlocks, kRadix, 0, s>>>(b.blocks, b.hist, b.segB);
+  const int nSeg = b.E * k2dLayers;
+  k_segsort2d<kSegThreadsS, 0, kSegSmall><<<nSeg, kSegThreadsS, 0, s>>>(b.evOff, b.segB, b.lg, b.scr, b.keyS, b.xyw, b.table,
+                                                      b.tableStride);
+  k_segsort2d<kSegThreadsL, kSegSmall, kSegCap><<<nSeg, kSegThreadsL, 0, s>>>(b.evOff, b.segB, b.lg, b.scr, b.keyS, b.xyw, b.table,
+                                                             b.tableStride);
+  k_segsort2d<kSegThreadsH, kSegCap, kSegNone><<<nSeg, kSegThreadsH, 0, s>>>(b.evOff, b.segB, b.lg, b.scr, b.keyS, b.xyw, b.table,
+                                                            b.tableStride);
+  const uint32_t* keyS = b.keyS;
   const int nTiles = b.nBlocks * kTilesPerSort;
   k_reach2d<<<(nTiles + kReachThreads - 1) / kReachThreads, kReachThreads, 0, s>>>(
       b.blocks, nTiles, keyS, b.table, b.tableStride, prm.dc, prm.outlierDeltaFactor, b.reachD, b.reachN);
   launch_density2d(b.densityVariant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.table, b.tableStride,
                    (const int4*)b.reachD, prm.dc, prm.dc2, b.rhoKey, io.rho);
-  int* parent = reinterpret_cast<int*>(b.k2);
+  int* parent = reinterpret_cast<int*>(b.lg);
   launch_nh2d(b.nhVariant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.rhoKey, b.table, b.tableStride,
               (const int4*)b.reachN, b.evOff, prm.dc, prm.rhoc, prm.outlierDeltaFactor, prm.dm2, prm.dm2up, io.delta,
               io.nearestHigher, io.isSeed, parent);
@@ -124,8 +119,8 @@ void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DPara
 void clue2d_enqueue_step(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DParams& prm, int which, int variant,
                          hipStream_t s) {
   if (b.N <= 0 || b.nBlocks <= 0) return;
-  const uint32_t* keyS = b.k1;
-  int* parent = reinterpret_cast<int*>(b.k2);
+  const uint32_t* keyS = b.keyS;
+  int* parent = reinterpret_cast<int*>(b.lg);
   if (which == 0)
     launch_density2d(variant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.table, b.tableStride, (const int4*)b.reachD,
                      prm.dc, prm.dc2, b.rhoKey, io.rho);
@@ -148,10 +143,10 @@ void clue2d_carve(void* base, int64_t N, int E, int nBlocks, Clue2DBuffers& b) {
   b.E = E;
   b.N = N;
   b.nBlocks = nBlocks;
-  b.k1 = cv.take<uint32_t>(n);
-  b.i1 = cv.take<int>(n);
-  b.k2 = cv.take<uint32_t>(n);
-  b.i2 = cv.take<int>(n);
+  b.keyS = cv.take<uint32_t>(n);
+  b.scr = cv.take<int>(n);
+  b.lg = cv.take<float4>(n);
+  b.segB = cv.take<int>((size_t)(E > 0 ? E : 1) * (k2dLayers + 1));
   b.xyw = cv.take<float4>(n);
   b.rhoKey = cv.take<uint32_t>(n);
   b.reachD = cv.take<int4>((size_t)(nBlocks > 0 ? nBlocks : 1) * kTilesPerSort);
@@ -161,7 +156,6 @@ void clue2d_carve(void* base, int64_t N, int E, int nBlocks, Clue2DBuffers& b) {
   b.part = cv.take<int>((size_t)scan_partials(maxScan > 0 ? maxScan : 1));
   b.tableStride = clue2d_table_stride();
   b.table = cv.take<int>((size_t)b.tableStride * (E > 0 ? E : 1));
-  b.tileStart = cv.take<int>((size_t)(table_tiles(k2dKeys) + 1) * (E > 0 ? E : 1));
   b.evOff = cv.take<int>((size_t)E + 1);
   b.blocks = cv.take<BlockDesc>((size_t)(nBlocks > 0 ? nBlocks : 1));
   b.seeds = seed_rank_carve(cv, N);
@@ -171,7 +165,10 @@ size_t clue2d_workspace_bytes(int64_t N, int nEvents, int nBlocks) {
   const size_t n = (size_t)(N > 0 ? N : 1);
   size_t bytes = 0;
   auto add = [&](size_t sz) { bytes += (sz + 255) & ~(size_t)255; };
-  for (int k = 0; k < 4; ++k) add(n * 4);                   // k1 i1 k2 i2
+  add(n * 4);                                               // keyS
+  add(n * 4);                                               // scr
+  add(n * 16);                                              // lg
+  add((size_t)(nEvents > 0 ? nEvents : 1) * (k2dLayers + 1) * 4);  // segB
   add(n * 16);                                              // xyw
   add(n * 4);                                               // rhoKey
   for (int k = 0; k < 2; ++k) add((size_t)(nBlocks > 0 ? nBlocks : 1) * kTilesPerSort * sizeof(int4));  // reach
@@ -179,7 +176,6 @@ size_t clue2d_workspace_bytes(int64_t N, int nEvents, int nBlocks) {
   const int64_t maxScan = (int64_t)kRadix * nBlocks > N ? (int64_t)kRadix * nBlocks : N;
   add((size_t)scan_partials(maxScan > 0 ? maxScan : 1) * 4);  // partials
   add((size_t)clue2d_table_stride() * (size_t)(nEvents > 0 ? nEvents : 1) * 4);
-  add((size_t)(table_tiles(k2dKeys) + 1) * (size_t)(nEvents > 0 ? nEvents : 1) * 4);  // tileStart
   add((size_t)(nEvents + 1) * 4);                                   // evOff
   add((size_t)(nBlocks > 0 ? nBlocks : 1) * sizeof(BlockDesc));     // blocks
   bytes += seed_rank_bytes(N);                                      // seed words

@@ -21,13 +21,21 @@
 namespace clue {
 namespace g2d {
 
-// Bin key of point i of event ev; an out-of-range layer sets kErrLayerRange in err[ev * errStride]
-// (errStride 0: one flag word for the whole batch).
-struct Keys2D {
-  const float* x;
-  const float* y;
+// ---------------------------------------------------------------- tile order
+//
+// The sorted array lists every (event, layer, xBin, yBin) bin in ascending point index. It is built
+// in two steps:
+//  (1) a stable counting-sort pass by layer (the radix-pass kernels of tiles.h with the layer as the
+//      digit), whose output is each point as {x, y, w, index bits}, grouped by (event, layer) and in
+//      index order inside a layer (k_layer_starts2d then lists where each layer begins);
+//  (2) k_segsort2d: one workgroup per (event, layer) orders that layer's points by bin, keeping index
+//      order inside a bin, writes the sorted keys and {x, y, w, index} records, and writes the
+//      layer's 10^4 entries of the event's bin table from its own bin counts.
+
+// Layer of point i of event ev, the digit of step (1); an out-of-range layer sets kErrLayerRange in
+// err[ev * errStride] (errStride 0: one flag word for the whole batch) and is clamped.
+struct LayerOf2D {
   const int32_t* layer;
-  uint32_t* keyOut;
   int* err;
   int errStride;
   __device__ __forceinline__ uint32_t operator()(int i, int ev) const {
@@ -36,35 +44,286 @@ struct Keys2D {
       atomicOr(err + ev * errStride, kErrLayerRange);
       L = L < 0 ? 0 : k2dLayers - 1;
     }
-    const uint32_t k = (uint32_t)(L * k2dBinsPerLayer + xbin2d(x[i]) * k2dRows + ybin2d(y[i]));
-    keyOut[i] = k;
-    return k;
+    return (uint32_t)L;
   }
 };
+using DigitLayer2D = Digit<1, kRadix>;
 
-// Last pass: also gathers x, y, weight and the point index into tile order as {x, y, w, index bits}
-// (the permutation itself is not stored).
-struct EmitLast2D {
-  const int* iIn;
-  uint32_t* kOut;
+// Output of step (1): only the point index passes through the block's LDS reorder; x, y and w are
+// then read from the block's own input range (consecutive points of one event) and written as one
+// 16-byte record at the point's position in layer order.
+struct EmitSeg2D {
   const float* x;
   const float* y;
   const float* w;
-  float4* xyw;
-  // the point index passes through LDS; x, y and w are gathered once it is in sorted order
+  float4* lg;
   using Payload = int;
   using Out = float4;
-  __device__ __forceinline__ Payload load(int i) const { return iIn[i]; }
+  __device__ __forceinline__ Payload load(int i) const { return i; }
   __device__ __forceinline__ Out fetch(int idx) const { return make_float4(x[idx], y[idx], w[idx], __int_as_float(idx)); }
-  __device__ __forceinline__ void store(int pos, uint32_t k, const Out& v) const {
-    kOut[pos] = k;
-    xyw[pos] = v;
+  __device__ __forceinline__ void store(int pos, uint32_t, const Out& v) const { lg[pos] = v; }
+};
+
+// segB[e * (k2dLayers + 1) + L] = batch position of event e's first point of layer L, L in [0, 100]
+// (entry 100: the event's end). After the exclusive scan, hist[histBase + L * nbEvent] of the event's
+// first block is the position where digit L of the event begins; digit 100 has no points, so its
+// entry is where the event ends. Empty events have no blocks and get no entries.
+__global__ __launch_bounds__(kRadix) void k_layer_starts2d(const BlockDesc* __restrict__ blocks,
+                                                           const int* __restrict__ histScanned,
+                                                           int* __restrict__ segB) {
+  const BlockDesc bd = blocks[blockIdx.x];
+  const int L = (int)threadIdx.x;
+  if (bd.bLocal != 0 || L > k2dLayers) return;
+  segB[bd.event * (k2dLayers + 1) + L] = histScanned[bd.histBase + L * bd.nbEvent];
+}
+
+// Segment size classes of k_segsort2d: up to kSegSmall points (kSegThreadsS threads, 28 KB of LDS,
+// 5 workgroups per CU), up to kSegCap points (kSegThreadsL threads, 72 KB, 2 per CU), and larger
+// ones (kSegThreadsH threads, slots in global memory). Every class runs over all (event, layer)
+// segments and leaves the others at once.
+constexpr int kSegSmall = 2048;
+constexpr int kSegCap = 10240;
+constexpr int kSegThreadsS = 256;
+constexpr int kSegThreadsL = 1024;
+constexpr int kSegThreadsH = 512;
+constexpr int kSegNone = 0;  // "no upper bound": the class of the segments above kSegCap
+constexpr int kSegIlp = 4;   // points of a thread taken together in each loop over a segment
+
+__device__ __forceinline__ int bin2d(const float4& v) { return xbin2d(v.x) * k2dRows + ybin2d(v.y); }
+
+// Exclusive prefix sum over the NT threads of a block (`s_tmp`: NT / kWave ints); two barriers, call
+// from uniform control flow.
+template <int NT>
+__device__ __forceinline__ int block_exclusive_sum_nt(int v, int* s_tmp) {
+  constexpr int W = NT / kWave;
+  const int lane = lane_id(), wave = (int)threadIdx.x / kWave;
+  const int inc = wave_inclusive_sum(v);
+  if (lane == kWave - 1) s_tmp[wave] = inc;
+  __syncthreads();
+  int base = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) base += w < wave ? s_tmp[w] : 0;
+  __syncthreads();
+  return base + inc - v;
+}
+
+using v4i = int __attribute__((ext_vector_type(4)));
+
+// Bin counts of a segment in LDS. LDS class (CAP > 0): two 16-bit counts per 32-bit word (a segment
+// has at most CAP < 65536 points), each thread owning SCAN consecutive bins (a multiple of 8, so its
+// bins are whole 16-byte words); the 32-bit atomics add 1 << 16 for an odd bin, which never carries
+// into the neighbouring count. Global class: one 32-bit count per bin.
+template <int NT, int CAP>
+struct SegCounts {
+  static constexpr bool kPacked = CAP != kSegNone;
+  static constexpr int kScan = kPacked ? (k2dBinsPerLayer + NT * 8 - 1) / (NT * 8) * 8 : 0;  // bins per thread
+  static constexpr int kWords = kPacked ? NT * kScan / 2 : k2dBinsPerLayer;                 // 32-bit words
+  static_assert(!kPacked || CAP < 65536, "16-bit counts");
+  static_assert(kWords % 4 == 0, "whole 16-byte words");
+  uint32_t* w;
+  __device__ __forceinline__ int get(int b) const {
+    if constexpr (kPacked)
+      return (int)((w[b >> 1] >> (16 * (b & 1))) & 0xFFFFu);
+    else
+      return (int)w[b];
+  }
+  // adds one to bin b and returns its previous value
+  __device__ __forceinline__ int take(int b) const {
+    if constexpr (kPacked) {
+      const int sh = 16 * (b & 1);
+      return (int)((atomicAdd(&w[b >> 1], 1u << sh) >> sh) & 0xFFFFu);
+    } else {
+      return (int)atomicAdd(&w[b], 1u);
+    }
   }
 };
 
-using DigitY = Digit<1, k2dRows>;
-using DigitX = Digit<k2dRows, k2dCols>;
-using DigitL = Digit<k2dBinsPerLayer, kRadix>;
+// Step (2) for the layer segment lg[start, end) of event e = blockIdx.x / 100, layer L = blockIdx.x %
+// 100, if its size n is in this class, (LO, CAP] (CAP = kSegNone: above LO). LayerTilesSerial::fill
+// (LayerTilesSerial.h:22) appends in index order, and the segment arrives in index order:
+//   counts of the 10^4 bins (LDS atomics); their exclusive scan is each bin's first position, written
+//   as the layer's bin-table entries; every point then takes the next free slot of its bin (LDS
+//   atomic, so the order inside a bin is arbitrary); finally each slot's point counts the points of
+//   its bin with a smaller position, which is its rank in index order, and is written at bin start +
+//   rank. The ranking costs the sum over bins of (points in the bin)^2, the order of the density
+//   step's own work on that bin. The LDS classes keep each point's bin and each slot's point in LDS
+//   (u16); the class above kSegCap keeps the slots in `scr` (the segment's own range of a global
+//   array) and recomputes bins. Empty segments take the smallest class: their table entries are
+//   written too.
+template <int NT, int LO, int CAP>
+__global__ __launch_bounds__(NT) void k_segsort2d(const int* __restrict__ evOff, const int* __restrict__ segB,
+                                                  const float4* __restrict__ lg, int* __restrict__ scr,
+                                                  uint32_t* __restrict__ keyOut, float4* __restrict__ xyw,
+                                                  int* __restrict__ table, int64_t stride) {
+  using C = SegCounts<NT, CAP>;
+  constexpr bool kLds = CAP != kSegNone;
+  constexpr int kPts = kLds ? CAP : 1;
+  __shared__ __align__(16) uint32_t s_w[C::kWords];
+  __shared__ uint16_t s_bin[kPts];
+  __shared__ uint16_t s_slot[kPts];
+  __shared__ int s_tmp[NT / kWave];
+  const int e = (int)blockIdx.x / k2dLayers, L = (int)blockIdx.x % k2dLayers;
+  const int evEnd = evOff[e + 1];
+  if (evOff[e] == evEnd) return;  // an empty event: nothing reads its table (uniform over the block)
+  const int start = segB[e * (k2dLayers + 1) + L];
+  const int n = segB[e * (k2dLayers + 1) + L + 1] - start;
+  if (n < LO || (LO > 0 && n == LO) || (kLds && n > CAP)) return;  // another class (uniform)
+  const C cnt{s_w};
+  const int tid = (int)threadIdx.x;
+  {
+    uint4* w4 = reinterpret_cast<uint4*>(s_w);
+    for (int k = tid; k < C::kWords / 4; k += NT) w4[k] = make_uint4(0u, 0u, 0u, 0u);
+  }
+  __syncthreads();
+  // bin counts; every loop over the points takes kSegIlp of a thread's points at once, all their
+  // loads issued before any is used
+  for (int p0 = tid; p0 < n; p0 += kSegIlp * NT) {
+    float4 v[kSegIlp];
+#pragma unroll
+    for (int r = 0; r < kSegIlp; ++r) {
+      const int p = p0 + r * NT;
+      v[r] = lg[start + (p < n ? p : 0)];
+    }
+#pragma unroll
+    for (int r = 0; r < kSegIlp; ++r) {
+      const int p = p0 + r * NT;
+      if (p < n) {
+        const int b = bin2d(v[r]);
+        if constexpr (kLds) s_bin[p] = (uint16_t)b;
+        cnt.take(b);
+      }
+    }
+  }
+  __syncthreads();
+  // exclusive scan of the counts
+  if constexpr (kLds) {
+    // thread t owns bins [t * kScan, (t + 1) * kScan): kScan / 8 whole 16-byte words
+    constexpr int kV = C::kScan / 8;
+    uint4* w4 = reinterpret_cast<uint4*>(s_w) + tid * kV;
+    uint4 v[kV];
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < kV; ++k) {
+      v[k] = w4[k];
+      const uint32_t q[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) sum += (int)(q[j] & 0xFFFFu) + (int)(q[j] >> 16);
+    }
+    uint32_t run = (uint32_t)block_exclusive_sum_nt<NT>(sum, s_tmp);
+#pragma unroll
+    for (int k = 0; k < kV; ++k) {
+      uint32_t q[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t lo = run, hi = run + (q[j] & 0xFFFFu);
+        run = hi + (q[j] >> 16);
+        q[j] = lo | (hi << 16);
+      }
+      w4[k] = make_uint4(q[0], q[1], q[2], q[3]);
+    }
+  } else {
+    constexpr int kScan = (k2dBinsPerLayer + NT - 1) / NT;
+    const int b0 = tid * kScan;
+    int sum = 0;
+    for (int k = 0; k < kScan; ++k)
+      if (b0 + k < k2dBinsPerLayer) sum += (int)s_w[b0 + k];
+    uint32_t run = (uint32_t)block_exclusive_sum_nt<NT>(sum, s_tmp);
+    for (int k = 0; k < kScan; ++k)
+      if (b0 + k < k2dBinsPerLayer) {
+        const uint32_t c = s_w[b0 + k];
+        s_w[b0 + k] = run;
+        run += c;
+      }
+  }
+  __syncthreads();
+  // the layer's bin-table entries, four per 16-byte store: the first sorted position of every bin
+  // (the entry after the last layer, the event's end, closes the table); full-line writes read back
+  // sparsely, kept out of L2
+  {
+    int* T = table + (int64_t)e * stride + (int64_t)L * k2dBinsPerLayer;  // 16-byte aligned
+    v4i* T4 = reinterpret_cast<v4i*>(T);
+    static_assert(k2dBinsPerLayer % 4 == 0, "whole 16-byte table words per layer");
+    for (int k = tid; k < k2dBinsPerLayer / 4; k += NT) {
+      v4i v;
+      if constexpr (C::kPacked) {
+        const uint2 q = reinterpret_cast<const uint2*>(s_w)[k];  // bins 4k .. 4k + 3
+        v.x = start + (int)(q.x & 0xFFFFu);
+        v.y = start + (int)(q.x >> 16);
+        v.z = start + (int)(q.y & 0xFFFFu);
+        v.w = start + (int)(q.y >> 16);
+      } else {
+        v.x = start + cnt.get(4 * k);
+        v.y = start + cnt.get(4 * k + 1);
+        v.z = start + cnt.get(4 * k + 2);
+        v.w = start + cnt.get(4 * k + 3);
+      }
+      __builtin_nontemporal_store(v, T4 + k);
+    }
+    if (L == k2dLayers - 1 && tid == 0) T[k2dBinsPerLayer] = evEnd;
+  }
+  __syncthreads();  // the bin starts are read above before the slots advance them
+  // slots
+  for (int p0 = tid; p0 < n; p0 += kSegIlp * NT) {
+    int b[kSegIlp];
+#pragma unroll
+    for (int r = 0; r < kSegIlp; ++r) {
+      const int p = p0 + r * NT;
+      if constexpr (kLds)
+        b[r] = p < n ? (int)s_bin[p] : 0;
+      else
+        b[r] = bin2d(lg[start + (p < n ? p : 0)]);
+    }
+#pragma unroll
+    for (int r = 0; r < kSegIlp; ++r) {
+      const int p = p0 + r * NT;
+      if (p < n) {
+        const int slot = cnt.take(b[r]);
+        if constexpr (kLds)
+          s_slot[slot] = (uint16_t)p;
+        else
+          scr[start + slot] = p;
+      }
+    }
+  }
+  __syncthreads();
+  // the count words now hold each bin's end; rank each slot's point inside its bin by position
+  for (int j0 = tid; j0 < n; j0 += kSegIlp * NT) {
+    int p[kSegIlp];
+    float4 v[kSegIlp];
+#pragma unroll
+    for (int r = 0; r < kSegIlp; ++r) {
+      const int j = j0 + r * NT < n ? j0 + r * NT : 0;
+      if constexpr (kLds)
+        p[r] = (int)s_slot[j];
+      else
+        p[r] = scr[start + j];
+    }
+#pragma unroll
+    for (int r = 0; r < kSegIlp; ++r) v[r] = lg[start + p[r]];
+#pragma unroll
+    for (int r = 0; r < kSegIlp; ++r) {
+      if (j0 + r * NT >= n) continue;
+      int b;
+      if constexpr (kLds)
+        b = (int)s_bin[p[r]];
+      else
+        b = bin2d(v[r]);
+      const int bs = b > 0 ? cnt.get(b - 1) : 0, be = cnt.get(b);
+      int rank = 0;
+      for (int f = bs; f < be; ++f) {
+        int q;
+        if constexpr (kLds)
+          q = (int)s_slot[f];
+        else
+          q = scr[start + f];
+        rank += q < p[r] ? 1 : 0;
+      }
+      const int out = start + bs + rank;
+      keyOut[out] = (uint32_t)(L * k2dBinsPerLayer + b);
+      xyw[out] = v[r];
+    }
+  }
+}
 
 // The density and nearest-higher kernels give each block ITEMS * kThreads consecutive sorted points
 // of one event (ITEMS points per thread). In x-major tile order everything their search boxes can

@@ -1,5 +1,5 @@
-// Runs the CLUE2D global-sort path (three radix passes, bin table, density, nearest higher, seed
-// ranks, labels) on the CPU through the HIP-model emulation, with clue2d_enqueue's launch sequence
+// Runs the CLUE2D global-sort path (layer pass, per-layer bin sort with the bin table, density,
+// nearest higher, seed ranks, labels) on the CPU through the HIP-model emulation, with clue2d_enqueue's launch sequence
 // and grids. Test infrastructure only.
 // usage: global2d_emu <in> <out> [density variant] [nearest-higher variant] (formats: launch_emu.h;
 // variants index kVariants2D)
@@ -37,14 +37,14 @@ int main(int argc, char** argv) {
   std::vector<int> evOff(E + 1);
   for (int e = 0; e <= E; ++e) evOff[e] = (int)in.offs[e];
   const int64_t stride = clue2d_table_stride();
-  const int tiles = table_tiles(k2dKeys);
   // device scratch starts with arbitrary contents
   const int P = 0x5A5A5A5A;
-  std::vector<uint32_t> k1(N, P), k2(N, P);
-  std::vector<int> i1(N, P), i2(N, P), hist((size_t)kRadix * std::max(nb, 1), P);
+  std::vector<uint32_t> keyS(N, P);
+  std::vector<int> scr(N, P), hist((size_t)kRadix * std::max(nb, 1), P), segB((size_t)std::max(E, 1) * (k2dLayers + 1), P);
+  std::vector<float4> lg(N, float4{NAN, NAN, NAN, NAN});
   const int64_t maxScan = std::max<int64_t>((int64_t)kRadix * nb, N);
   std::vector<int> part(scan_partials(std::max<int64_t>(maxScan, 1)), P);
-  std::vector<int> table((size_t)stride * std::max(E, 1), P), tileStart((size_t)(tiles + 1) * std::max(E, 1), P);
+  std::vector<int> table((size_t)stride * std::max(E, 1), P);
   std::vector<float4> xyw(N, float4{NAN, NAN, NAN, NAN});
   std::vector<uint32_t> rhoKey(N, 0x5A5A5A5Au);
   std::vector<float> rho(N, -7.f), delta(N, -7.f);
@@ -54,44 +54,42 @@ int main(int argc, char** argv) {
   const float dc2 = sqrt_le(dc), dm2 = sqrt_le(odf * dc), dm2up = std::nextafter(dm2, INFINITY);
   if (nb > 0) {
     const int nHist = kRadix * nb;
-    Keys2D keys{in.x.data(), in.y.data(), in.layer.data(), k2.data(), &err, 0};
-    emu::launch(nb, kThreads, [&] { k_radix_hist<Keys2D, DigitY>(blocks.data(), keys, hist.data()); });
+    const LayerOf2D layerOf{in.layer.data(), &err, 0};
+    emu::launch(nb, kThreads, [&] { k_radix_hist<LayerOf2D, DigitLayer2D>(blocks.data(), layerOf, hist.data()); });
     scan_exclusive(hist.data(), hist.data(), nHist, part.data());
     emu::launch(nb, kThreads, [&] {
-      k_radix_scatter<KeyArray, DigitY, EmitFirst>(blocks.data(), KeyArray{k2.data()}, hist.data(),
-                                                   EmitFirst{k1.data(), i1.data()});
+      k_radix_scatter<LayerOf2D, DigitLayer2D, EmitSeg2D>(blocks.data(), layerOf, hist.data(),
+                                                          EmitSeg2D{in.x.data(), in.y.data(), in.w.data(), lg.data()});
     });
-    emu::launch(nb, kThreads, [&] { k_radix_hist<KeyArray, DigitX>(blocks.data(), KeyArray{k1.data()}, hist.data()); });
-    scan_exclusive(hist.data(), hist.data(), nHist, part.data());
-    emu::launch(nb, kThreads, [&] {
-      k_radix_scatter<KeyArray, DigitX, EmitMid>(blocks.data(), KeyArray{k1.data()}, hist.data(),
-                                                 EmitMid{i1.data(), k2.data(), i2.data()});
+    emu::launch(nb, kRadix, [&] { k_layer_starts2d(blocks.data(), hist.data(), segB.data()); });
+    emu::launch(E * k2dLayers, kSegThreadsS, [&] {
+      k_segsort2d<kSegThreadsS, 0, kSegSmall>(evOff.data(), segB.data(), lg.data(), scr.data(), keyS.data(), xyw.data(),
+                                     table.data(), stride);
     });
-    emu::launch(nb, kThreads, [&] { k_radix_hist<KeyArray, DigitL>(blocks.data(), KeyArray{k2.data()}, hist.data()); });
-    scan_exclusive(hist.data(), hist.data(), nHist, part.data());
-    emu::launch(nb, kThreads, [&] {
-      k_radix_scatter<KeyArray, DigitL, EmitLast2D>(
-          blocks.data(), KeyArray{k2.data()}, hist.data(),
-          EmitLast2D{i2.data(), k1.data(), in.x.data(), in.y.data(), in.w.data(), xyw.data()});
+    emu::launch(E * k2dLayers, kSegThreadsL, [&] {
+      k_segsort2d<kSegThreadsL, kSegSmall, kSegCap>(evOff.data(), segB.data(), lg.data(), scr.data(), keyS.data(), xyw.data(),
+                                           table.data(), stride);
     });
-    build_table(k1.data(), blocks.data(), nb, evOff.data(), E, k2dKeys, k2dBinsPerLayer, 0, stride, table.data(),
-                tileStart.data());
+    emu::launch(E * k2dLayers, kSegThreadsH, [&] {
+      k_segsort2d<kSegThreadsH, kSegCap, kSegNone>(evOff.data(), segB.data(), lg.data(), scr.data(), keyS.data(), xyw.data(),
+                                          table.data(), stride);
+    });
     const int nTiles = nb * kTilesPerSort;
     std::vector<int4> reachD(nTiles, int4{P, P, P, P}), reachN(nTiles, int4{P, P, P, P});
     emu::launch((nTiles + kReachThreads - 1) / kReachThreads, kReachThreads, [&] {
-      k_reach2d(blocks.data(), nTiles, k1.data(), table.data(), stride, dc, odf, reachD.data(), reachN.data());
+      k_reach2d(blocks.data(), nTiles, keyS.data(), table.data(), stride, dc, odf, reachD.data(), reachN.data());
     });
     const int dv = argc > 3 ? atoi(argv[3]) : 0, nv = argc > 4 ? atoi(argv[4]) : 0;
     dispatch(dv, [&](auto I, auto S) {
       emu::launch(CompShape<I>::kPerSort * nb, kThreads, [&] {
-        k_density2d<I, S>(blocks.data(), k1.data(), xyw.data(), table.data(), stride, reachD.data(), dc, dc2,
+        k_density2d<I, S>(blocks.data(), keyS.data(), xyw.data(), table.data(), stride, reachD.data(), dc, dc2,
                           rhoKey.data(), rho.data());
       });
     });
-    int* parent = reinterpret_cast<int*>(k2.data());
+    int* parent = reinterpret_cast<int*>(lg.data());
     dispatch(nv, [&](auto I, auto S) {
       emu::launch(CompShape<I>::kPerSort * nb, kThreads, [&] {
-        k_nh2d<I, S>(blocks.data(), k1.data(), xyw.data(), rhoKey.data(), table.data(), stride, reachN.data(),
+        k_nh2d<I, S>(blocks.data(), keyS.data(), xyw.data(), rhoKey.data(), table.data(), stride, reachN.data(),
                      evOff.data(), dc, rhoc, odf, dm2, dm2up, delta.data(), nh.data(), isSeed.data(), parent);
       });
     });

@@ -125,6 +125,33 @@ def test_emulated_global_batch(emulator, tmp_path, oracle_lib):
     helpers.assert_bit_exact(got, oracle_lib.clue2d_batch(packed, offs), helpers.OUT2D, "emulated global batch")
 
 
+def large_layer_event(seed=43):
+    """One event with a layer in each size class of k_segsort2d: 12000 points on layer 3 (above the
+    10240 it keeps in LDS), 5000 on layer 50 (the middle class), both with one densely filled bin, and
+    300 on layers 0 and 99."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    xs, ys, ls = [], [], []
+    for lay, n, dense in ((3, 12000, 2000), (50, 5000, 700), (0, 300, 0), (99, 300, 0)):
+        xs += [rng.uniform(-250, 250, n - dense), rng.uniform(10.0, 14.9, dense)]
+        ys += [rng.uniform(-250, 250, n - dense), rng.uniform(-4.9, -0.1, dense)]
+        ls.append(np.full(n, lay))
+    x = np.concatenate(xs).astype(np.float32)
+    y = np.concatenate(ys).astype(np.float32)
+    layer = np.concatenate(ls).astype(np.int32)
+    perm = rng.permutation(x.size)
+    return {"x": x[perm], "y": y[perm], "layer": layer[perm], "weight": rng.exponential(1.0, x.size).astype(np.float32)}
+
+
+@pytest.mark.parametrize("exe", ["global2d_emu", "global2d_emu_ldsz", "global2d_emu_tsan"])
+def test_emulated_global_large_layer(emulator, tmp_path, oracle_lib, exe):
+    """Layers of every size class of the per-layer bin sort, the largest on its global-slot path."""
+    ev = large_layer_event()
+    err, _, got = run_emulated(EMU / "_build" / exe, tmp_path, ev, np.array([0, ev["x"].size]))
+    assert err == 0
+    want = oracle_lib.clue2d(ev["x"], ev["y"], ev["layer"], ev["weight"])
+    helpers.assert_bit_exact(got, want, helpers.OUT2D, f"emulated large layer {exe}")
+
+
 @pytest.mark.parametrize("variants", [(1, 1), (2, 3), (3, 2)])
 def test_emulated_global_variants(emulator, tmp_path, variants):
     """Every (points per workgroup, LDS staging) variant of density and nearest higher gives the same bits."""
