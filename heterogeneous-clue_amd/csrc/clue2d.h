@@ -41,7 +41,8 @@ struct Clue2DBuffers {
   int4* reachN = nullptr;  // the same for the nearest-higher boxes
   int* hist = nullptr;
   int* part = nullptr;
-  int* table = nullptr;
+  uint16_t* table16 = nullptr;  // per event: bin-table entries as offsets from their layer's start
+  int* table = nullptr;         // per event: the entries of layers too large for 16-bit offsets
   int64_t tableStride = 0;
   SeedRank seeds{};
   int* err = nullptr;  // device error flags: err[event * errStride]

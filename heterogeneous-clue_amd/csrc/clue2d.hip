@@ -97,22 +97,23 @@ void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DPara
       b.blocks, layerOf, b.hist, EmitSeg2D{io.x, io.y, io.weight, b.lg});
   k_layer_starts2d<<<b.nBlocks, kRadix, 0, s>>>(b.blocks, b.hist, b.segB);
   const int nSeg = b.E * k2dLayers;
-  k_segsort2d<kSegThreadsS, 0, kSegSmall><<<nSeg, kSegThreadsS, 0, s>>>(b.evOff, b.segB, b.lg, b.scr, b.keyS, b.xyw, b.table,
-                                                      b.tableStride);
-  k_segsort2d<kSegThreadsL, kSegSmall, kSegCap><<<nSeg, kSegThreadsL, 0, s>>>(b.evOff, b.segB, b.lg, b.scr, b.keyS, b.xyw, b.table,
-                                                             b.tableStride);
-  k_segsort2d<kSegThreadsH, kSegCap, kSegNone><<<nSeg, kSegThreadsH, 0, s>>>(b.evOff, b.segB, b.lg, b.scr, b.keyS, b.xyw, b.table,
-                                                            b.tableStride);
+  k_segsort2d<kSegThreadsS, 0, kSegSmall><<<nSeg, kSegThreadsS, 0, s>>>(b.evOff, b.segB, b.lg, b.scr, b.keyS, b.xyw, b.table16,
+                                                      b.table, b.tableStride);
+  k_segsort2d<kSegThreadsL, kSegSmall, kSegCap><<<nSeg, kSegThreadsL, 0, s>>>(b.evOff, b.segB, b.lg, b.scr, b.keyS, b.xyw, b.table16,
+                                                             b.table, b.tableStride);
+  k_segsort2d<kSegThreadsH, kSegCap, kSegNone><<<nSeg, kSegThreadsH, 0, s>>>(b.evOff, b.segB, b.lg, b.scr, b.keyS, b.xyw, b.table16,
+                                                            b.table, b.tableStride);
   const uint32_t* keyS = b.keyS;
+  const BinTables2D tabs{b.table16, b.table, b.segB, b.tableStride};
   const int nTiles = b.nBlocks * kTilesPerSort;
   k_reach2d<<<(nTiles + kReachThreads - 1) / kReachThreads, kReachThreads, 0, s>>>(
-      b.blocks, nTiles, keyS, b.table, b.tableStride, prm.dc, prm.outlierDeltaFactor, b.reachD, b.reachN);
-  launch_density2d(b.densityVariant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.table, b.tableStride,
-                   (const int4*)b.reachD, prm.dc, prm.dc2, b.rhoKey, io.rho);
+      b.blocks, nTiles, keyS, tabs, prm.dc, prm.outlierDeltaFactor, b.reachD, b.reachN);
+  launch_density2d(b.densityVariant, b.nBlocks, s, b.blocks, keyS, b.xyw, tabs, (const int4*)b.reachD, prm.dc,
+                   prm.dc2, b.rhoKey, io.rho);
   int* parent = reinterpret_cast<int*>(b.lg);
-  launch_nh2d(b.nhVariant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.rhoKey, b.table, b.tableStride,
-              (const int4*)b.reachN, b.evOff, prm.dc, prm.rhoc, prm.outlierDeltaFactor, prm.dm2, prm.dm2up, io.delta,
-              io.nearestHigher, io.isSeed, parent);
+  launch_nh2d(b.nhVariant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.rhoKey, tabs, (const int4*)b.reachN, b.evOff,
+              prm.dc, prm.rhoc, prm.outlierDeltaFactor, prm.dm2, prm.dm2up, io.delta, io.nearestHigher, io.isSeed,
+              parent);
   launch_labels(b.blocks, b.nBlocks, parent, io.isSeed, b.N, b.evOff, io.clusterIndex, b.seeds, b.part, s);
 }
 
@@ -120,13 +121,13 @@ void clue2d_enqueue_step(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2
                          hipStream_t s) {
   if (b.N <= 0 || b.nBlocks <= 0) return;
   const uint32_t* keyS = b.keyS;
+  const BinTables2D tabs{b.table16, b.table, b.segB, b.tableStride};
   int* parent = reinterpret_cast<int*>(b.lg);
   if (which == 0)
-    launch_density2d(variant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.table, b.tableStride, (const int4*)b.reachD,
+    launch_density2d(variant, b.nBlocks, s, b.blocks, keyS, b.xyw, tabs, (const int4*)b.reachD,
                      prm.dc, prm.dc2, b.rhoKey, io.rho);
   else
-    launch_nh2d(variant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.rhoKey, b.table, b.tableStride,
-                (const int4*)b.reachN, b.evOff, prm.dc, prm.rhoc, prm.outlierDeltaFactor, prm.dm2, prm.dm2up, io.delta,
+    launch_nh2d(variant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.rhoKey, tabs, (const int4*)b.reachN, b.evOff, prm.dc, prm.rhoc, prm.outlierDeltaFactor, prm.dm2, prm.dm2up, io.delta,
                 io.nearestHigher, io.isSeed, parent);
 }
 
@@ -155,6 +156,7 @@ void clue2d_carve(void* base, int64_t N, int E, int nBlocks, Clue2DBuffers& b) {
   const int64_t maxScan = (int64_t)kRadix * nBlocks > N ? (int64_t)kRadix * nBlocks : N;
   b.part = cv.take<int>((size_t)scan_partials(maxScan > 0 ? maxScan : 1));
   b.tableStride = clue2d_table_stride();
+  b.table16 = cv.take<uint16_t>((size_t)b.tableStride * (E > 0 ? E : 1));
   b.table = cv.take<int>((size_t)b.tableStride * (E > 0 ? E : 1));
   b.evOff = cv.take<int>((size_t)E + 1);
   b.blocks = cv.take<BlockDesc>((size_t)(nBlocks > 0 ? nBlocks : 1));
@@ -175,7 +177,8 @@ size_t clue2d_workspace_bytes(int64_t N, int nEvents, int nBlocks) {
   add((size_t)kRadix * (size_t)(nBlocks > 0 ? nBlocks : 1) * 4);  // hist
   const int64_t maxScan = (int64_t)kRadix * nBlocks > N ? (int64_t)kRadix * nBlocks : N;
   add((size_t)scan_partials(maxScan > 0 ? maxScan : 1) * 4);  // partials
-  add((size_t)clue2d_table_stride() * (size_t)(nEvents > 0 ? nEvents : 1) * 4);
+  add((size_t)clue2d_table_stride() * (size_t)(nEvents > 0 ? nEvents : 1) * 2);  // table16
+  add((size_t)clue2d_table_stride() * (size_t)(nEvents > 0 ? nEvents : 1) * 4);  // table
   add((size_t)(nEvents + 1) * 4);                                   // evOff
   add((size_t)(nBlocks > 0 ? nBlocks : 1) * sizeof(BlockDesc));     // blocks
   bytes += seed_rank_bytes(N);                                      // seed words

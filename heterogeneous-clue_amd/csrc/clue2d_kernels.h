@@ -91,6 +91,58 @@ constexpr int kSegIlp = 4;   // points of a thread taken together in each loop o
 
 __device__ __forceinline__ int bin2d(const float4& v) { return xbin2d(v.x) * k2dRows + ybin2d(v.y); }
 
+// The bin table of an event: entry b (key b = layer * 10^4 + bin, b <= k2dKeys) is the first sorted
+// position whose key is >= b. Stored as a 16-bit offset from the start of b's layer (segB) when the
+// layer has at most kTab16Max points, else as a 32-bit position in t32 (written only for such
+// layers); the closing entry b = k2dKeys is the event's end (offset 0 from segB[k2dLayers]).
+constexpr int kTab16Max = 65535;
+// The entries of one layer L: j in [0, 10^4] is entry L * 10^4 + j (j = 10^4, the next layer's first
+// entry, is where layer L ends).
+struct LayerTab2D {
+  const uint16_t* t16;
+  const int* t32;
+  int off;  // L * 10^4
+  int base, next;
+  bool wide;
+  __device__ __forceinline__ int operator[](int j) const {
+    const int v = wide ? t32[off + j] : base + (int)t16[off + j];
+    return j < k2dBinsPerLayer ? v : next;
+  }
+  // the same for a layer of 16-bit entries: one load whose value is always used, so that a point's
+  // loads are issued together (entry j = 10^4 is the next layer's bin 0, offset 0 from `next`)
+  __device__ __forceinline__ int narrow(int j) const {
+    return (j < k2dBinsPerLayer ? base : next) + (int)t16[off + j];
+  }
+};
+struct BinTable2D {
+  const uint16_t* t16;
+  const int* t32;
+  const int* segB;  // the event's k2dLayers + 1 layer starts
+  // any entry: the 16-bit word is loaded together with the layer starts; the 32-bit one only for
+  // a layer that uses it
+  __device__ __forceinline__ int operator[](int b) const {
+    const int L = b / k2dBinsPerLayer;
+    const int s = segB[L], e = segB[L < k2dLayers ? L + 1 : L];
+    const int v16 = s + (int)t16[b];
+    if (e - s <= kTab16Max) return v16;
+    return t32[b];
+  }
+  __device__ __forceinline__ LayerTab2D layer(int L) const {
+    const int s = segB[L], e = segB[L + 1];
+    return LayerTab2D{t16 + L * k2dBinsPerLayer, t32 + L * k2dBinsPerLayer, 0, s, e, e - s > kTab16Max};
+  }
+};
+// The bin tables of a batch (one row of `stride` entries per event in each array).
+struct BinTables2D {
+  const uint16_t* t16;
+  const int* t32;
+  const int* segB;
+  int64_t stride;
+  __device__ __forceinline__ BinTable2D event(int e) const {
+    return BinTable2D{t16 + (int64_t)e * stride, t32 + (int64_t)e * stride, segB + e * (k2dLayers + 1)};
+  }
+};
+
 // Exclusive prefix sum over the NT threads of a block (`s_tmp`: NT / kWave ints); two barriers, call
 // from uniform control flow.
 template <int NT>
@@ -154,7 +206,7 @@ template <int NT, int LO, int CAP>
 __global__ __launch_bounds__(NT) void k_segsort2d(const int* __restrict__ evOff, const int* __restrict__ segB,
                                                   const float4* __restrict__ lg, int* __restrict__ scr,
                                                   uint32_t* __restrict__ keyOut, float4* __restrict__ xyw,
-                                                  int* __restrict__ table, int64_t stride) {
+                                                  uint16_t* __restrict__ t16, int* __restrict__ t32, int64_t stride) {
   using C = SegCounts<NT, CAP>;
   constexpr bool kLds = CAP != kSegNone;
   constexpr int kPts = kLds ? CAP : 1;
@@ -236,30 +288,28 @@ __global__ __launch_bounds__(NT) void k_segsort2d(const int* __restrict__ evOff,
       }
   }
   __syncthreads();
-  // the layer's bin-table entries, four per 16-byte store: the first sorted position of every bin
-  // (the entry after the last layer, the event's end, closes the table); full-line writes read back
-  // sparsely, kept out of L2
+  // the layer's bin-table entries: each bin's first position as an offset from the layer's start
+  // (in the LDS classes the scanned count words already hold them in that layer-row format, two
+  // 16-bit entries per word); the entry after the last layer closes the table. Full-line writes
+  // read back sparsely, kept out of L2.
   {
-    int* T = table + (int64_t)e * stride + (int64_t)L * k2dBinsPerLayer;  // 16-byte aligned
-    v4i* T4 = reinterpret_cast<v4i*>(T);
-    static_assert(k2dBinsPerLayer % 4 == 0, "whole 16-byte table words per layer");
-    for (int k = tid; k < k2dBinsPerLayer / 4; k += NT) {
-      v4i v;
-      if constexpr (C::kPacked) {
-        const uint2 q = reinterpret_cast<const uint2*>(s_w)[k];  // bins 4k .. 4k + 3
-        v.x = start + (int)(q.x & 0xFFFFu);
-        v.y = start + (int)(q.x >> 16);
-        v.z = start + (int)(q.y & 0xFFFFu);
-        v.w = start + (int)(q.y >> 16);
-      } else {
-        v.x = start + cnt.get(4 * k);
-        v.y = start + cnt.get(4 * k + 1);
-        v.z = start + cnt.get(4 * k + 2);
-        v.w = start + cnt.get(4 * k + 3);
-      }
-      __builtin_nontemporal_store(v, T4 + k);
+    const int64_t row = (int64_t)e * stride + (int64_t)L * k2dBinsPerLayer;  // 16-byte aligned
+    static_assert(k2dBinsPerLayer % 8 == 0, "whole 16-byte table words per layer");
+    if constexpr (C::kPacked) {
+      using v4u = unsigned __attribute__((ext_vector_type(4)));
+      const v4u* w4 = reinterpret_cast<const v4u*>(s_w);
+      v4u* D = reinterpret_cast<v4u*>(t16 + row);
+      for (int k = tid; k < k2dBinsPerLayer / 8; k += NT) __builtin_nontemporal_store(w4[k], D + k);
+    } else if (n <= kTab16Max) {
+      for (int b = tid; b < k2dBinsPerLayer; b += NT) __builtin_nontemporal_store((uint16_t)cnt.get(b), t16 + row + b);
+    } else {
+      for (int b = tid; b < k2dBinsPerLayer; b += NT) __builtin_nontemporal_store(start + cnt.get(b), t32 + row + b);
+      if (tid == 0) t16[row] = 0;  // bin 0 starts the layer: read as the previous layer's end entry
     }
-    if (L == k2dLayers - 1 && tid == 0) T[k2dBinsPerLayer] = evEnd;
+    if (L == k2dLayers - 1 && tid == 0) {
+      t16[(int64_t)e * stride + k2dKeys] = 0;
+      t32[(int64_t)e * stride + k2dKeys] = evEnd;
+    }
   }
   __syncthreads();  // the bin starts are read above before the slots advance them
   // slots
@@ -342,6 +392,11 @@ struct CompShape {
 };
 constexpr int kMaxCols = 5;  // column runs kept in registers per point
 
+// Waves per SIMD the compiler must allow for the variants that hold one or two points per thread
+// with the 1024-point stage (their LDS admits 8 and 7): it keeps their registers to 64 and 72.
+template <int ITEMS, int STAGE>
+constexpr int kWavesPerEu2D = ITEMS == 1 ? 8 : (ITEMS == 2 && STAGE == 1024) ? 7 : 1;
+
 struct Variant2D {
   int items;  // points per thread
   int stage;  // staged points
@@ -360,27 +415,45 @@ struct Runs {
 // The event's bin table, read from the block's LDS copy of entries [base, base + n) when the block
 // staged them, else from global memory (uniform over the block).
 struct TableView {
-  const int* g;
+  BinTable2D g;
   const int* s;
   int base;
   bool staged;
-  __device__ __forceinline__ int operator[](int b) const { return staged ? s[b - base] : g[b]; }
 };
 
-template <class Tab>
-__device__ __forceinline__ void make_runs(const Tab& T, uint32_t key, float xi, float yi, float d, Runs& r) {
+__device__ __forceinline__ void make_runs(const TableView& T, uint32_t key, float xi, float yi, float d, Runs& r) {
   r.L = (int)(key / (uint32_t)k2dBinsPerLayer);
   r.xlo = xbin2d(xi - d);
   r.xhi = xbin2d(xi + d);
   r.ylo = ybin2d(yi - d);
   r.yhi = ybin2d(yi + d);
   r.n = r.xhi - r.xlo + 1;
-  const int L0 = r.L * k2dBinsPerLayer;
+  if (T.staged) {
+    const int* S = T.s + (r.L * k2dBinsPerLayer - T.base);
 #pragma unroll
-  for (int k = 0; k < kMaxCols; ++k) {
-    const int xb = r.xlo + k < r.xhi ? r.xlo + k : r.xhi;
-    r.s[k] = T[L0 + xb * k2dRows + r.ylo];
-    r.t[k] = T[L0 + xb * k2dRows + r.yhi + 1];
+    for (int k = 0; k < kMaxCols; ++k) {
+      const int xb = r.xlo + k < r.xhi ? r.xlo + k : r.xhi;
+      r.s[k] = S[xb * k2dRows + r.ylo];
+      r.t[k] = S[xb * k2dRows + r.yhi + 1];
+    }
+  } else {
+    // every lookup of a box stays in the point's own layer: the ten 16-bit loads are issued together
+    // with the layer's starts, and only a wave with a lane in a layer of 32-bit entries reads those
+    const LayerTab2D LT = T.g.layer(r.L);
+#pragma unroll
+    for (int k = 0; k < kMaxCols; ++k) {
+      const int xb = r.xlo + k < r.xhi ? r.xlo + k : r.xhi;
+      r.s[k] = LT.narrow(xb * k2dRows + r.ylo);
+      r.t[k] = LT.narrow(xb * k2dRows + r.yhi + 1);
+    }
+    if (__ballot(LT.wide) != 0) {
+#pragma unroll
+      for (int k = 0; k < kMaxCols; ++k) {
+        const int xb = r.xlo + k < r.xhi ? r.xlo + k : r.xhi;
+        r.s[k] = LT[xb * k2dRows + r.ylo];
+        r.t[k] = LT[xb * k2dRows + r.yhi + 1];
+      }
+    }
   }
 }
 
@@ -412,9 +485,8 @@ __device__ __forceinline__ void reach_columns(uint32_t k0, uint32_t k1, float d,
 constexpr int kTilesPerSort = kSortBlock / kThreads;
 constexpr int kReachThreads = 64;  // one wave per block: the few hundred blocks spread over all CUs
 __global__ __launch_bounds__(kReachThreads) void k_reach2d(const BlockDesc* __restrict__ blocks, int nTiles,
-                                                           const uint32_t* __restrict__ keyS,
-                                                           const int* __restrict__ table, int64_t stride, float dc,
-                                                           float odf, int4* __restrict__ reachD,
+                                                           const uint32_t* __restrict__ keyS, BinTables2D tabs,
+                                                           float dc, float odf, int4* __restrict__ reachD,
                                                            int4* __restrict__ reachN) {
   const int tile = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (tile >= nTiles) return;
@@ -422,7 +494,7 @@ __global__ __launch_bounds__(kReachThreads) void k_reach2d(const BlockDesc* __re
   const int b0 = bd.begin + (tile % kTilesPerSort) * kThreads;
   if (b0 >= bd.end) return;  // past the event's last point: no compute block reads it
   const int b1 = b0 + kThreads < bd.end ? b0 + kThreads : bd.end;
-  const int* T = table + (int64_t)bd.event * stride;
+  const BinTable2D T = tabs.event(bd.event);
   const uint32_t k0 = keyS[b0], k1 = keyS[b1 - 1];
   int tloD, thiD, tloN, thiN;
   reach_columns(k0, k1, dc, tloD, thiD);
@@ -448,13 +520,26 @@ __device__ __forceinline__ void compute_reach(const int4* __restrict__ reach, in
 
 // Entries [tlo, tlo + tn) of the table into LDS (tn <= kTabStage), all loads of a thread in flight.
 constexpr int kTabStage = 768;
-__device__ __forceinline__ void stage_table(const int* __restrict__ T, int* st, int tlo, int tn) {
+__device__ __forceinline__ void stage_table(const BinTable2D& T, int* st, int tlo, int tn) {
   constexpr int kPer = kTabStage / kThreads;
-  int v[kPer];
+  int b[kPer], v[kPer];
+  bool wide[kPer];
+  bool anyWide = false;
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
     const int k = u * kThreads + (int)threadIdx.x;
-    v[u] = T[tlo + (k < tn ? k : tn - 1)];
+    b[u] = tlo + (k < tn ? k : tn - 1);
+    const int L = b[u] / k2dBinsPerLayer;
+    const int s = T.segB[L], e = T.segB[L < k2dLayers ? L + 1 : L];
+    v[u] = s + (int)T.t16[b[u]];
+    wide[u] = e - s > kTab16Max;
+    anyWide |= wide[u];
+  }
+  // the 32-bit entries of a layer too large for 16-bit offsets, only when a lane of the wave needs one
+  if (__ballot(anyWide) != 0) {
+#pragma unroll
+    for (int u = 0; u < kPer; ++u)
+      if (wide[u]) v[u] = T.t32[b[u]];
   }
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
@@ -534,10 +619,11 @@ __device__ __forceinline__ void stage_range_nh(const NhSrc& g, float4* st, int l
 }
 
 // Run k >= kMaxCols of a box: sorted positions [s, t) of column xlo + k, read from the table.
-__device__ __forceinline__ void run_bounds(const int* __restrict__ T, const Runs& r, int k, int& s, int& t) {
-  const int* TL = T + r.L * k2dBinsPerLayer + (r.xlo + k) * k2dRows;
-  s = TL[r.ylo];
-  t = TL[r.yhi + 1];
+__device__ __forceinline__ void run_bounds(const BinTable2D& T, const Runs& r, int k, int& s, int& t) {
+  const LayerTab2D LT = T.layer(r.L);
+  const int j = (r.xlo + k) * k2dRows;
+  s = LT[j + r.ylo];
+  t = LT[j + r.yhi + 1];
 }
 
 // Visits the candidates of the register runs [s[k], t[k]), k < min(n, kMaxCols), in column order,
@@ -622,10 +708,9 @@ __device__ __forceinline__ float density_run(int p, float xi, float yi, int s, i
 }
 
 template <int ITEMS, int STAGE>
-__global__ __launch_bounds__(kThreads) void k_density2d(const BlockDesc* __restrict__ blocks,
+__global__ __launch_bounds__(kThreads) CLUE_WAVES_PER_EU(kWavesPerEu2D<ITEMS, STAGE>) void k_density2d(const BlockDesc* __restrict__ blocks,
                                                         const uint32_t* __restrict__ keyS,
-                                                        const float4* __restrict__ xyw,
-                                                        const int* __restrict__ table, int64_t stride,
+                                                        const float4* __restrict__ xyw, BinTables2D tabs,
                                                         const int4* __restrict__ reach, float dc, float dc2,
                                                         uint32_t* __restrict__ rhoKey, float* __restrict__ rhoOut) {
   using Shape = CompShape<ITEMS>;
@@ -637,7 +722,7 @@ __global__ __launch_bounds__(kThreads) void k_density2d(const BlockDesc* __restr
   const int b0 = bd.begin + (blk % Shape::kPerSort) * Shape::kHits;
   const int b1 = (b0 + Shape::kHits) < bd.end ? (b0 + Shape::kHits) : bd.end;
   if (b0 >= b1) return;  // uniform over the block
-  const int* T = table + (int64_t)bd.event * stride;
+  const BinTable2D T = tabs.event(bd.event);
   int rlo, rhi, tlo, tn;
   compute_reach<ITEMS>(reach, blk, b0, b1, rlo, rhi, tlo, tn);
   const bool staged = rhi - rlo <= STAGE;
@@ -758,7 +843,7 @@ __device__ __forceinline__ void nh_run_exact(uint64_t qkey, float xi, float yi, 
 }
 
 template <bool STAGED>
-__device__ __forceinline__ int nearest_higher_of(uint64_t qkey, float xi, float yi, const int* __restrict__ T,
+__device__ __forceinline__ int nearest_higher_of(uint64_t qkey, float xi, float yi, const BinTable2D& T,
                                                  const Runs& runs, const NhSrc& src, const float4* st, int lo,
                                                  float dm2, float dm2up, float& delta) {
   NhStep<STAGED> f{qkey, xi, yi, src, st, lo, dm2up, INFINITY, -1};
@@ -787,10 +872,10 @@ __device__ __forceinline__ int nearest_higher_of(uint64_t qkey, float xi, float 
 // delta / nearestHigher plus the seed/outlier decision of kernel_findAndAssign_clusters
 // (CLUEAlgoKernels.h:108-137).
 template <int ITEMS, int STAGE>
-__global__ __launch_bounds__(kThreads) void k_nh2d(const BlockDesc* __restrict__ blocks,
+__global__ __launch_bounds__(kThreads) CLUE_WAVES_PER_EU(kWavesPerEu2D<ITEMS, STAGE>) void k_nh2d(const BlockDesc* __restrict__ blocks,
                                                    const uint32_t* __restrict__ keyS,
                                                    const float4* __restrict__ xyw, const uint32_t* __restrict__ rhoKey,
-                                                   const int* __restrict__ table, int64_t stride,
+                                                   BinTables2D tabs,
                                                    const int4* __restrict__ reach, const int* __restrict__ evOff,
                                                    float dc, float rhoc,
                                                    float odf, float dm2, float dm2up, float* __restrict__ deltaOut,
@@ -805,7 +890,7 @@ __global__ __launch_bounds__(kThreads) void k_nh2d(const BlockDesc* __restrict__
   const int b0 = bd.begin + (blk % Shape::kPerSort) * Shape::kHits;
   const int b1 = (b0 + Shape::kHits) < bd.end ? (b0 + Shape::kHits) : bd.end;
   if (b0 >= b1) return;
-  const int* T = table + (int64_t)bd.event * stride;
+  const BinTable2D T = tabs.event(bd.event);
   const int evBase = evOff[bd.event];
   const float dm = odf * dc;
   const NhSrc src{xyw, rhoKey};

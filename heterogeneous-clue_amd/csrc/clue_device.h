@@ -11,6 +11,12 @@
 
 #pragma clang fp contract(off)
 
+// Minimum waves per SIMD a kernel's register allocation must admit (the CPU emulation of the HIP
+// model defines it away).
+#ifndef CLUE_WAVES_PER_EU
+#define CLUE_WAVES_PER_EU(...) __attribute__((amdgpu_waves_per_eu(__VA_ARGS__)))
+#endif
+
 namespace clue {
 
 // Device-side error bits, OR-ed into the context's flag word and reported after the run.

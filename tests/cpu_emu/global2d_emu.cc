@@ -45,6 +45,8 @@ int main(int argc, char** argv) {
   const int64_t maxScan = std::max<int64_t>((int64_t)kRadix * nb, N);
   std::vector<int> part(scan_partials(std::max<int64_t>(maxScan, 1)), P);
   std::vector<int> table((size_t)stride * std::max(E, 1), P);
+  std::vector<uint16_t> table16((size_t)stride * std::max(E, 1), 0x5A5A);
+  const BinTables2D tabs{table16.data(), table.data(), segB.data(), stride};
   std::vector<float4> xyw(N, float4{NAN, NAN, NAN, NAN});
   std::vector<uint32_t> rhoKey(N, 0x5A5A5A5Au);
   std::vector<float> rho(N, -7.f), delta(N, -7.f);
@@ -64,32 +66,32 @@ int main(int argc, char** argv) {
     emu::launch(nb, kRadix, [&] { k_layer_starts2d(blocks.data(), hist.data(), segB.data()); });
     emu::launch(E * k2dLayers, kSegThreadsS, [&] {
       k_segsort2d<kSegThreadsS, 0, kSegSmall>(evOff.data(), segB.data(), lg.data(), scr.data(), keyS.data(), xyw.data(),
-                                     table.data(), stride);
+                                     table16.data(), table.data(), stride);
     });
     emu::launch(E * k2dLayers, kSegThreadsL, [&] {
       k_segsort2d<kSegThreadsL, kSegSmall, kSegCap>(evOff.data(), segB.data(), lg.data(), scr.data(), keyS.data(), xyw.data(),
-                                           table.data(), stride);
+                                           table16.data(), table.data(), stride);
     });
     emu::launch(E * k2dLayers, kSegThreadsH, [&] {
       k_segsort2d<kSegThreadsH, kSegCap, kSegNone>(evOff.data(), segB.data(), lg.data(), scr.data(), keyS.data(), xyw.data(),
-                                          table.data(), stride);
+                                          table16.data(), table.data(), stride);
     });
     const int nTiles = nb * kTilesPerSort;
     std::vector<int4> reachD(nTiles, int4{P, P, P, P}), reachN(nTiles, int4{P, P, P, P});
     emu::launch((nTiles + kReachThreads - 1) / kReachThreads, kReachThreads, [&] {
-      k_reach2d(blocks.data(), nTiles, keyS.data(), table.data(), stride, dc, odf, reachD.data(), reachN.data());
+      k_reach2d(blocks.data(), nTiles, keyS.data(), tabs, dc, odf, reachD.data(), reachN.data());
     });
     const int dv = argc > 3 ? atoi(argv[3]) : 0, nv = argc > 4 ? atoi(argv[4]) : 0;
     dispatch(dv, [&](auto I, auto S) {
       emu::launch(CompShape<I>::kPerSort * nb, kThreads, [&] {
-        k_density2d<I, S>(blocks.data(), keyS.data(), xyw.data(), table.data(), stride, reachD.data(), dc, dc2,
+        k_density2d<I, S>(blocks.data(), keyS.data(), xyw.data(), tabs, reachD.data(), dc, dc2,
                           rhoKey.data(), rho.data());
       });
     });
     int* parent = reinterpret_cast<int*>(lg.data());
     dispatch(nv, [&](auto I, auto S) {
       emu::launch(CompShape<I>::kPerSort * nb, kThreads, [&] {
-        k_nh2d<I, S>(blocks.data(), keyS.data(), xyw.data(), rhoKey.data(), table.data(), stride, reachN.data(),
+        k_nh2d<I, S>(blocks.data(), keyS.data(), xyw.data(), rhoKey.data(), tabs, reachN.data(),
                      evOff.data(), dc, rhoc, odf, dm2, dm2up, delta.data(), nh.data(), isSeed.data(), parent);
       });
     });

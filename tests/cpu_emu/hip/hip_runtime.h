@@ -23,6 +23,7 @@
 #define __host__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+#define CLUE_WAVES_PER_EU(...)
 #ifdef EMU_LDS_REDZONES
 // Every `__shared__` array a plain static: AddressSanitizer puts a redzone after each one, so an
 // index past the end of an LDS array stops the run here (on the device such an access is dropped

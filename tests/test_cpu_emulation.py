@@ -152,6 +152,21 @@ def test_emulated_global_large_layer(emulator, tmp_path, oracle_lib, exe):
     helpers.assert_bit_exact(got, want, helpers.OUT2D, f"emulated large layer {exe}")
 
 
+def test_emulated_global_wide_layer(emulator, tmp_path, oracle_lib):
+    """A layer of 70,000 points: its bin-table entries exceed 16-bit offsets from the layer's start and
+    take the 32-bit table, next to a small layer that keeps the 16-bit one."""
+    rng = np.random.Generator(np.random.PCG64(44))
+    n = 70000
+    ev = {"x": np.concatenate([rng.uniform(-250, 250, n), rng.uniform(-250, 250, 500)]).astype(np.float32),
+          "y": np.concatenate([rng.uniform(-250, 250, n), rng.uniform(-250, 250, 500)]).astype(np.float32),
+          "layer": np.concatenate([np.full(n, 7), np.full(500, 8)]).astype(np.int32)}
+    ev["weight"] = rng.exponential(1.0, ev["x"].size).astype(np.float32)
+    err, _, got = run_emulated(EXE_GLOBAL, tmp_path, ev, np.array([0, ev["x"].size]))
+    assert err == 0
+    want = oracle_lib.clue2d(ev["x"], ev["y"], ev["layer"], ev["weight"])
+    helpers.assert_bit_exact(got, want, helpers.OUT2D, "emulated wide layer")
+
+
 @pytest.mark.parametrize("variants", [(1, 1), (2, 3), (3, 2)])
 def test_emulated_global_variants(emulator, tmp_path, variants):
     """Every (points per workgroup, LDS staging) variant of density and nearest higher gives the same bits."""
