@@ -1,5 +1,6 @@
-// CLUE2D on gfx950: tile build, density, nearest-higher (+ seed/outlier flags) and labels for a
-// packed batch of events. A single event is the batch of one.
+// CLUE2D on gfx950: tile order (layer pass, per-layer bin sort with the bin table), density,
+// nearest-higher (+ seed/outlier flags) and labels for a packed batch of events. A single event is
+// the batch of one.
 //
 // Reference semantics: src/serial/plugin-CLUEClusterizer/CLUEAlgoKernels.h:7-153.
 // Bit-exactness rests on three facts:

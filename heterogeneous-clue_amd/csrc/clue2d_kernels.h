@@ -1,6 +1,7 @@
-// Device code of the CLUE2D global-sort path: key and payload functors of the three radix passes
-// and the density / nearest-higher kernels. Header-only so that the kernels also build for the CPU
-// emulation of the HIP execution model used by the tests; the launch sequence is in clue2d.hip.
+// Device code of the CLUE2D global-sort path: the layer pass functors, the per-layer bin sort that
+// writes the bin table, and the density / nearest-higher kernels. Header-only so that the kernels
+// also build for the CPU emulation of the HIP execution model used by the tests; the launch sequence
+// is in clue2d.hip.
 //
 // Reference semantics: src/serial/plugin-CLUEClusterizer/CLUEAlgoKernels.h:7-153.
 // Bit-exactness rests on three facts:
