@@ -619,6 +619,25 @@ __device__ __forceinline__ void stage_range_nh(const NhSrc& g, float4* st, int l
   }
 }
 
+// stage_range for the density candidates: {x, y, 0.5f * w, index} (see DensityStaged).
+template <int STAGE>
+__device__ __forceinline__ void stage_range_half(const float4* __restrict__ g, float4* st, int lo, int hi) {
+  static_assert(STAGE % kThreads == 0, "staging splits evenly over the block");
+  constexpr int kPer = STAGE / kThreads;
+  const int n = hi - lo;
+  float4 v[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int k = u * kThreads + (int)threadIdx.x;
+    v[u] = g[lo + (k < n ? k : n - 1)];
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int k = u * kThreads + (int)threadIdx.x;
+    if (k < n) st[k] = make_float4(v[u].x, v[u].y, 0.5f * v[u].z, v[u].w);
+  }
+}
+
 // Run k >= kMaxCols of a box: sorted positions [s, t) of column xlo + k, read from the table.
 __device__ __forceinline__ void run_bounds(const BinTable2D& T, const Runs& r, int k, int& s, int& t) {
   const LayerTab2D LT = T.layer(r.L);
@@ -653,6 +672,80 @@ __device__ __forceinline__ void walk_runs(const Runs& r, F& f) {
     }
   }
 }
+
+// The same walk over staged candidates, with less work per step. A run is packed into one word as
+// LDS byte offsets from the start of the staging array (low 16 bits: first candidate, high 16 bits:
+// one past the last; a stage holds at most 2048 points of 16 bytes, so both fit); 0 is the empty
+// run. The runs are compacted once per point (empty ones dropped, order kept), so that a run switch
+// is a single shift of the pending words; a step reads the candidates at qa and qa + 16 (the second
+// masked off at an odd run end: the staging array has one spare element after its last), and the
+// lane leaves the loop when the shift brings in an empty run.
+constexpr int kStageMax = 2048;
+struct StagedRuns {
+  uint32_t r0, r1, r2, r3, r4;
+};
+__device__ __forceinline__ uint32_t pack_run(int s, int t, int lo) {
+  return t > s ? ((uint32_t)(s - lo) << 4) | ((uint32_t)(t - lo) << 20) : 0u;
+}
+__device__ __forceinline__ StagedRuns pack_runs(const Runs& r, int lo) {
+  uint32_t v[kMaxCols];
+#pragma unroll
+  for (int k = 0; k < kMaxCols; ++k) v[k] = k < r.n ? pack_run(r.s[k], r.t[k], lo) : 0u;
+  // stable compaction: each pass moves every empty run one place towards the end
+#pragma unroll
+  for (int i = 0; i < kMaxCols - 1; ++i) {
+#pragma unroll
+    for (int j = 0; j < kMaxCols - 1 - i; ++j) {
+      const bool sw = v[j] == 0u;
+      const uint32_t a = v[j], b = v[j + 1];
+      v[j] = sw ? b : a;
+      v[j + 1] = sw ? a : b;
+    }
+  }
+  return StagedRuns{v[0], v[1], v[2], v[3], v[4]};
+}
+template <class F>
+__device__ __forceinline__ void walk_staged(const StagedRuns& R, F& f) {
+  int qa = (int)(R.r0 & 0xFFFFu), ta = (int)(R.r0 >> 16);
+  uint32_t p1 = R.r1, p2 = R.r2, p3 = R.r3, p4 = R.r4;
+  while (qa < ta) {
+    f(qa, qa + 16 < ta);
+    qa += 32;
+    if (qa >= ta) {
+      qa = (int)(p1 & 0xFFFFu);
+      ta = (int)(p1 >> 16);
+      p1 = p2;
+      p2 = p3;
+      p3 = p4;
+      p4 = 0u;
+    }
+  }
+}
+__device__ __forceinline__ float4 staged_at(const unsigned char* st, int qa) {
+  return *reinterpret_cast<const float4*>(st + qa);
+}
+
+// Density candidates at staged byte offsets qa and (if two) qa + 16. The density staging holds
+// {x, y, 0.5f * w, index}: the reference's weight factor (i == j ? 1.f : 0.5f) * w[j] is that
+// product for every other point and w itself for the point (wself, at byte offset self).
+struct DensityStaged {
+  float xi, yi, dc2, wself;
+  int self, selfBefore;  // the point's byte offset, and that minus 16
+  const unsigned char* st;
+  float rho;
+  __device__ __forceinline__ void operator()(int qa, bool two) {
+    const float4 c0 = staged_at(st, qa);
+    const float4 c1 = staged_at(st, qa + 16);
+    const float dx0 = xi - c0.x, dy0 = yi - c0.y;
+    const float dx1 = xi - c1.x, dy1 = yi - c1.y;
+    const float d20 = dx0 * dx0 + dy0 * dy0;
+    const float d21 = dx1 * dx1 + dy1 * dy1;
+    const float w0 = qa == self ? wself : c0.z;
+    const float w1 = qa == selfBefore ? wself : c1.z;
+    rho += d20 <= dc2 ? w0 : 0.f;
+    rho += (two & (d21 <= dc2)) ? w1 : 0.f;
+  }
+};
 
 // Candidates q0 and (if two) q1 = q0 + 1, in that order: their rho contributions
 // (CLUEAlgoKernels.h:36-52); see density_run. A masked or failing term adds +0.f.
@@ -715,7 +808,8 @@ __global__ __launch_bounds__(kThreads) CLUE_WAVES_PER_EU(kWavesPerEu2D<ITEMS, ST
                                                         const int4* __restrict__ reach, float dc, float dc2,
                                                         uint32_t* __restrict__ rhoKey, float* __restrict__ rhoOut) {
   using Shape = CompShape<ITEMS>;
-  __shared__ float4 s_st[STAGE];
+  static_assert(STAGE <= kStageMax, "staged byte offsets take 16 bits");
+  __shared__ float4 s_st[STAGE + 1];  // + the spare element a masked second candidate may read
   __shared__ int s_tab[kTabStage];
   // consecutive compute blocks (overlapping staged ranges) run on one XCD and share its L2
   const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
@@ -727,7 +821,7 @@ __global__ __launch_bounds__(kThreads) CLUE_WAVES_PER_EU(kWavesPerEu2D<ITEMS, ST
   int rlo, rhi, tlo, tn;
   compute_reach<ITEMS>(reach, blk, b0, b1, rlo, rhi, tlo, tn);
   const bool staged = rhi - rlo <= STAGE;
-  if (staged) stage_range<STAGE>(xyw, s_st, rlo, rhi);
+  if (staged) stage_range_half<STAGE>(xyw, s_st, rlo, rhi);
   const TableView tab{T, s_tab, tlo, tn <= kTabStage};
   if (tab.staged) stage_table(T, s_tab, tlo, tn);
   float4 me[ITEMS];
@@ -749,19 +843,24 @@ __global__ __launch_bounds__(kThreads) CLUE_WAVES_PER_EU(kWavesPerEu2D<ITEMS, ST
       const float xi = me[r].x, yi = me[r].y;
       float rho = 0.f;
       if (staged) {
-        DensityStep<true> f{p, xi, yi, dc2, xyw, s_st, rlo, 0.f};
-        walk_runs(runs[r], f);
+        const int self = (p - rlo) * 16;
+        DensityStaged f{xi, yi, dc2, me[r].z, self, self - 16, reinterpret_cast<const unsigned char*>(s_st), 0.f};
+        walk_staged(pack_runs(runs[r], rlo), f);
+        for (int k = kMaxCols; k < runs[r].n; ++k) {
+          int s, t;
+          run_bounds(T, runs[r], k, s, t);
+          walk_staged(StagedRuns{pack_run(s, t, rlo), 0u, 0u, 0u, 0u}, f);
+        }
         rho = f.rho;
       } else {
         DensityStep<false> f{p, xi, yi, dc2, xyw, s_st, rlo, 0.f};
         walk_runs(runs[r], f);
         rho = f.rho;
-      }
-      for (int k = kMaxCols; k < runs[r].n; ++k) {
-        int s, t;
-        run_bounds(T, runs[r], k, s, t);
-        rho = staged ? density_run<true>(p, xi, yi, s, t, xyw, s_st, rlo, dc2, rho)
-                     : density_run<false>(p, xi, yi, s, t, xyw, s_st, rlo, dc2, rho);
+        for (int k = kMaxCols; k < runs[r].n; ++k) {
+          int s, t;
+          run_bounds(T, runs[r], k, s, t);
+          rho = density_run<false>(p, xi, yi, s, t, xyw, s_st, rlo, dc2, rho);
+        }
       }
       const int i = __float_as_int(me[r].w);
       rhoKey[p] = rho_key(rho);
@@ -825,6 +924,30 @@ struct NhStep {
   }
 };
 
+// NhStep over staged candidates at byte offsets qa and (if two) qa + 16 (see walk_staged).
+struct NhStaged {
+  uint64_t qkey;
+  float xi, yi;
+  const unsigned char* st;
+  float best2, prev2;
+  int best;
+  __device__ __forceinline__ void take(const float4& c, bool valid) {
+    const float dx = xi - c.x;
+    const float dy = yi - c.y;
+    const float d2 = dx * dx + dy * dy;
+    const bool upd = valid & (cand_key(c) > qkey) & (d2 < best2);
+    prev2 = upd ? best2 : prev2;
+    best2 = upd ? d2 : best2;
+    best = upd ? __float_as_int(c.z) : best;
+  }
+  __device__ __forceinline__ void operator()(int qa, bool two) {
+    const float4 c0 = staged_at(st, qa);
+    const float4 c1 = staged_at(st, qa + 16);
+    take(c0, true);
+    take(c1, two);
+  }
+};
+
 template <bool STAGED>
 __device__ __forceinline__ void nh_run_exact(uint64_t qkey, float xi, float yi, int s, int t, const NhSrc& src,
                                              const float4* st, int lo, float dm2, float& delta, int& nh) {
@@ -847,14 +970,30 @@ template <bool STAGED>
 __device__ __forceinline__ int nearest_higher_of(uint64_t qkey, float xi, float yi, const BinTable2D& T,
                                                  const Runs& runs, const NhSrc& src, const float4* st, int lo,
                                                  float dm2, float dm2up, float& delta) {
-  NhStep<STAGED> f{qkey, xi, yi, src, st, lo, dm2up, INFINITY, -1};
-  walk_runs(runs, f);
-  float best2 = f.best2, prev2 = f.prev2;
-  int best = f.best;
-  for (int k = kMaxCols; k < runs.n; ++k) {
-    int s, t;
-    run_bounds(T, runs, k, s, t);
-    nh_run<STAGED>(qkey, xi, yi, s, t, src, st, lo, best2, best, prev2);
+  float best2, prev2;
+  int best;
+  if constexpr (STAGED) {
+    NhStaged f{qkey, xi, yi, reinterpret_cast<const unsigned char*>(st), dm2up, INFINITY, -1};
+    walk_staged(pack_runs(runs, lo), f);
+    for (int k = kMaxCols; k < runs.n; ++k) {
+      int s, t;
+      run_bounds(T, runs, k, s, t);
+      walk_staged(StagedRuns{pack_run(s, t, lo), 0u, 0u, 0u, 0u}, f);
+    }
+    best2 = f.best2;
+    prev2 = f.prev2;
+    best = f.best;
+  } else {
+    NhStep<STAGED> f{qkey, xi, yi, src, st, lo, dm2up, INFINITY, -1};
+    walk_runs(runs, f);
+    best2 = f.best2;
+    prev2 = f.prev2;
+    best = f.best;
+    for (int k = kMaxCols; k < runs.n; ++k) {
+      int s, t;
+      run_bounds(T, runs, k, s, t);
+      nh_run<STAGED>(qkey, xi, yi, s, t, src, st, lo, best2, best, prev2);
+    }
   }
   delta = FLT_MAX;
   if (best < 0) return -1;
@@ -883,7 +1022,8 @@ __global__ __launch_bounds__(kThreads) CLUE_WAVES_PER_EU(kWavesPerEu2D<ITEMS, ST
                                                    int32_t* __restrict__ nhOut, int32_t* __restrict__ isSeedOut,
                                                    int* __restrict__ parent) {
   using Shape = CompShape<ITEMS>;
-  __shared__ float4 s_st[STAGE];
+  static_assert(STAGE <= kStageMax, "staged byte offsets take 16 bits");
+  __shared__ float4 s_st[STAGE + 1];  // + the spare element a masked second candidate may read
   __shared__ int s_tab[kTabStage];
   // consecutive compute blocks (overlapping staged ranges) run on one XCD and share its L2
   const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
