@@ -413,12 +413,15 @@ struct Runs {
   int n, L, xlo, xhi, ylo, yhi;
 };
 
-// The event's bin table, read from the block's LDS copy of entries [base, base + n) when the block
-// staged them, else from global memory (uniform over the block).
+// The event's bin table as a block reads it: when its window fits (staged, uniform over the block),
+// the raw 16-bit entries [base, base + n) and the starts of the window's layers from L0 on are in the
+// block's LDS, else everything comes from global memory. A point of a layer with 32-bit entries
+// (more than kTab16Max points) reads those from global memory either way.
 struct TableView {
   BinTable2D g;
-  const int* s;
-  int base;
+  const uint16_t* s;      // staged entry b at s[b - base]
+  const int* layerStart;  // staged start of layer L at layerStart[L - L0]
+  int base, L0;
   bool staged;
 };
 
@@ -429,13 +432,16 @@ __device__ __forceinline__ void make_runs(const TableView& T, uint32_t key, floa
   r.ylo = ybin2d(yi - d);
   r.yhi = ybin2d(yi + d);
   r.n = r.xhi - r.xlo + 1;
-  if (T.staged) {
-    const int* S = T.s + (r.L * k2dBinsPerLayer - T.base);
+  const int lb = T.staged ? T.layerStart[r.L - T.L0] : 0, nb = T.staged ? T.layerStart[r.L + 1 - T.L0] : 0;
+  if (T.staged && nb - lb <= kTab16Max) {
+    // entry j of the layer is its start + the 16-bit offset; j = 10^4 is the next layer's bin 0
+    const uint16_t* S = T.s + (r.L * k2dBinsPerLayer - T.base);
 #pragma unroll
     for (int k = 0; k < kMaxCols; ++k) {
       const int xb = r.xlo + k < r.xhi ? r.xlo + k : r.xhi;
-      r.s[k] = S[xb * k2dRows + r.ylo];
-      r.t[k] = S[xb * k2dRows + r.yhi + 1];
+      const int j = xb * k2dRows + r.yhi + 1;
+      r.s[k] = lb + (int)S[xb * k2dRows + r.ylo];
+      r.t[k] = (j < k2dBinsPerLayer ? lb : nb) + (int)S[j];
     }
   } else {
     // every lookup of a box stays in the point's own layer: the ten 16-bit loads are issued together
@@ -519,34 +525,30 @@ __device__ __forceinline__ void compute_reach(const int4* __restrict__ reach, in
   tn = r1.w - r0.z + 1;
 }
 
-// Entries [tlo, tlo + tn) of the table into LDS (tn <= kTabStage), all loads of a thread in flight.
-constexpr int kTabStage = 768;
-__device__ __forceinline__ void stage_table(const BinTable2D& T, int* st, int tlo, int tn) {
-  constexpr int kPer = kTabStage / kThreads;
-  int b[kPer], v[kPer];
-  bool wide[kPer];
-  bool anyWide = false;
-#pragma unroll
-  for (int u = 0; u < kPer; ++u) {
-    const int k = u * kThreads + (int)threadIdx.x;
-    b[u] = tlo + (k < tn ? k : tn - 1);
-    const int L = b[u] / k2dBinsPerLayer;
-    const int s = T.segB[L], e = T.segB[L < k2dLayers ? L + 1 : L];
-    v[u] = s + (int)T.t16[b[u]];
-    wide[u] = e - s > kTab16Max;
-    anyWide |= wide[u];
-  }
-  // the 32-bit entries of a layer too large for 16-bit offsets, only when a lane of the wave needs one
-  if (__ballot(anyWide) != 0) {
-#pragma unroll
-    for (int u = 0; u < kPer; ++u)
-      if (wide[u]) v[u] = T.t32[b[u]];
-  }
-#pragma unroll
-  for (int u = 0; u < kPer; ++u) {
-    const int k = u * kThreads + (int)threadIdx.x;
-    if (k < tn) st[k] = v[u];
-  }
+// The table window [tlo, tlo + tn) of a block: its raw 16-bit entries from the 16-byte word that
+// holds tlo (the rows of an event's table and each layer's 10^4 entries start on 16-byte words), as
+// one 16-byte load per thread, and the starts of its layers (up to 101, one load per thread). It is
+// staged when it spans at most kTab16Stage entries (3 KB of LDS).
+constexpr int kTab16Stage = 1536;
+static_assert(kTab16Stage % 8 == 0 && kTab16Stage / 8 <= kThreads && k2dLayers + 1 <= kThreads, "one load per thread");
+struct TabWindow {
+  int base, n, L0, nStarts;
+  bool fits;
+};
+__device__ __forceinline__ TabWindow table_window(int tlo, int tn) {
+  TabWindow w;
+  w.base = tlo & ~7;
+  w.n = tlo + tn - w.base;
+  w.fits = w.n <= kTab16Stage;
+  w.L0 = tlo / k2dBinsPerLayer;
+  const int Lend = (tlo + tn - 1) / k2dBinsPerLayer + 1;  // one past the last entry's layer
+  w.nStarts = (Lend < k2dLayers ? Lend : k2dLayers) - w.L0 + 1;
+  return w;
+}
+__device__ __forceinline__ void stage_table(const BinTable2D& T, const TabWindow& w, uint16_t* st, int* layerStart) {
+  const int tid = (int)threadIdx.x;
+  if (tid < (w.n + 7) / 8) reinterpret_cast<uint4*>(st)[tid] = reinterpret_cast<const uint4*>(T.t16 + w.base)[tid];
+  if (tid < w.nStarts) layerStart[tid] = T.segB[w.L0 + tid];
 }
 
 template <bool STAGED>
@@ -810,7 +812,8 @@ __global__ __launch_bounds__(kThreads) CLUE_WAVES_PER_EU(kWavesPerEu2D<ITEMS, ST
   using Shape = CompShape<ITEMS>;
   static_assert(STAGE <= kStageMax, "staged byte offsets take 16 bits");
   __shared__ float4 s_st[STAGE + 1];  // + the spare element a masked second candidate may read
-  __shared__ int s_tab[kTabStage];
+  __shared__ __align__(16) uint16_t s_tab[kTab16Stage];
+  __shared__ int s_layerStart[k2dLayers + 1];
   // consecutive compute blocks (overlapping staged ranges) run on one XCD and share its L2
   const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
   const BlockDesc bd = blocks[blk / Shape::kPerSort];
@@ -822,8 +825,9 @@ __global__ __launch_bounds__(kThreads) CLUE_WAVES_PER_EU(kWavesPerEu2D<ITEMS, ST
   compute_reach<ITEMS>(reach, blk, b0, b1, rlo, rhi, tlo, tn);
   const bool staged = rhi - rlo <= STAGE;
   if (staged) stage_range_half<STAGE>(xyw, s_st, rlo, rhi);
-  const TableView tab{T, s_tab, tlo, tn <= kTabStage};
-  if (tab.staged) stage_table(T, s_tab, tlo, tn);
+  const TabWindow win = table_window(tlo, tn);
+  const TableView tab{T, s_tab, s_layerStart, win.base, win.L0, win.fits};
+  if (tab.staged) stage_table(T, win, s_tab, s_layerStart);
   float4 me[ITEMS];
   uint32_t key[ITEMS];
 #pragma unroll
@@ -1024,7 +1028,8 @@ __global__ __launch_bounds__(kThreads) CLUE_WAVES_PER_EU(kWavesPerEu2D<ITEMS, ST
   using Shape = CompShape<ITEMS>;
   static_assert(STAGE <= kStageMax, "staged byte offsets take 16 bits");
   __shared__ float4 s_st[STAGE + 1];  // + the spare element a masked second candidate may read
-  __shared__ int s_tab[kTabStage];
+  __shared__ __align__(16) uint16_t s_tab[kTab16Stage];
+  __shared__ int s_layerStart[k2dLayers + 1];
   // consecutive compute blocks (overlapping staged ranges) run on one XCD and share its L2
   const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
   const BlockDesc bd = blocks[blk / Shape::kPerSort];
@@ -1039,8 +1044,9 @@ __global__ __launch_bounds__(kThreads) CLUE_WAVES_PER_EU(kWavesPerEu2D<ITEMS, ST
   compute_reach<ITEMS>(reach, blk, b0, b1, rlo, rhi, tlo, tn);
   const bool staged = rhi - rlo <= STAGE;
   if (staged) stage_range_nh<STAGE>(src, s_st, rlo, rhi);
-  const TableView tab{T, s_tab, tlo, tn <= kTabStage};
-  if (tab.staged) stage_table(T, s_tab, tlo, tn);
+  const TabWindow win = table_window(tlo, tn);
+  const TableView tab{T, s_tab, s_layerStart, win.base, win.L0, win.fits};
+  if (tab.staged) stage_table(T, win, s_tab, s_layerStart);
   float4 me[ITEMS];
   uint32_t key[ITEMS];
 #pragma unroll
