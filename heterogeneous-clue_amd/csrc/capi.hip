@@ -935,17 +935,19 @@ int32_t clue_plan_path(const clue_plan* pl) {
 
 int32_t clue_plan_tuning(const clue_plan* pl, int32_t* out4) {
   if (!pl || !out4) return CLUE_ERR_ARG;
-  int di = 0, ds = 0, ni = 0, ns = 0;
+  int dp = 0, ds = 0, np = 0, ns = 0;  // points per block and staged points of each kernel
   if (pl->dim == 3) {
-    clue3d_variant_shape(pl->b3.densityVariant, di, ds);
-    clue3d_variant_shape(pl->b3.nhVariant, ni, ns);
+    clue3d_variant_shape(pl->b3.densityVariant, dp, ds);
+    clue3d_variant_shape(pl->b3.nhVariant, np, ns);
+    dp *= kThreads;  // the CLUE3D kernels run 256-thread blocks of that many points per thread
+    np *= kThreads;
   } else {
-    clue2d_variant_shape(pl->b2.densityVariant, di, ds);
-    clue2d_variant_shape(pl->b2.nhVariant, ni, ns);
+    clue2d_variant_shape(pl->b2.densityVariant, dp, ds);
+    clue2d_variant_shape(pl->b2.nhVariant, np, ns);
   }
-  out4[0] = di * kThreads;
+  out4[0] = dp;
   out4[1] = ds;
-  out4[2] = ni * kThreads;
+  out4[2] = np;
   out4[3] = ns;
   return CLUE_OK;
 }

@@ -73,9 +73,9 @@ void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DPara
 // outputs with the same values, so the autotuner can time them in place.
 void clue2d_enqueue_step(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DParams& p, int which, int variant,
                          hipStream_t s);
-constexpr int kClue2DVariants = 4;
-// Points per thread and staged points of a variant.
-void clue2d_variant_shape(int variant, int& items, int& stage);
+constexpr int kClue2DVariants = 6;
+// Points per block and staged points of a variant.
+void clue2d_variant_shape(int variant, int& points, int& stage);
 size_t clue2d_workspace_bytes(int64_t N, int nEvents, int nBlocks);
 void clue2d_carve(void* base, int64_t N, int E, int nBlocks, Clue2DBuffers& b);
 

@@ -56,12 +56,12 @@ namespace {
 template <int V, class... A>
 void density_v(int nBlocks, hipStream_t s, A... a) {
   constexpr Variant2D v = kVariants2D[V];
-  k_density2d<v.items, v.stage><<<CompShape<v.items>::kPerSort * nBlocks, kThreads, 0, s>>>(a...);
+  k_density2d<v.threads, v.items, v.stage><<<CompShape<v.threads, v.items>::kPerSort * nBlocks, v.threads, 0, s>>>(a...);
 }
 template <int V, class... A>
 void nh_v(int nBlocks, hipStream_t s, A... a) {
   constexpr Variant2D v = kVariants2D[V];
-  k_nh2d<v.items, v.stage><<<CompShape<v.items>::kPerSort * nBlocks, kThreads, 0, s>>>(a...);
+  k_nh2d<v.threads, v.items, v.stage><<<CompShape<v.threads, v.items>::kPerSort * nBlocks, v.threads, 0, s>>>(a...);
 }
 
 }  // namespace
@@ -72,6 +72,8 @@ void launch_density2d(int variant, int nBlocks, hipStream_t s, A... a) {
     case 1: density_v<1>(nBlocks, s, a...); break;
     case 2: density_v<2>(nBlocks, s, a...); break;
     case 3: density_v<3>(nBlocks, s, a...); break;
+    case 4: density_v<4>(nBlocks, s, a...); break;
+    case 5: density_v<5>(nBlocks, s, a...); break;
     default: density_v<0>(nBlocks, s, a...); break;
   }
 }
@@ -81,10 +83,12 @@ void launch_nh2d(int variant, int nBlocks, hipStream_t s, A... a) {
     case 1: nh_v<1>(nBlocks, s, a...); break;
     case 2: nh_v<2>(nBlocks, s, a...); break;
     case 3: nh_v<3>(nBlocks, s, a...); break;
+    case 4: nh_v<4>(nBlocks, s, a...); break;
+    case 5: nh_v<5>(nBlocks, s, a...); break;
     default: nh_v<0>(nBlocks, s, a...); break;
   }
 }
-static_assert(kNumVariants2D == 4, "launch_density2d / launch_nh2d dispatch four variants");
+static_assert(kNumVariants2D == 6, "launch_density2d / launch_nh2d dispatch six variants");
 
 void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DParams& prm, hipStream_t s) {
   if (b.N <= 0 || b.nBlocks <= 0) return;
@@ -132,9 +136,9 @@ void clue2d_enqueue_step(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2
                 io.nearestHigher, io.isSeed, parent);
 }
 
-void clue2d_variant_shape(int variant, int& items, int& stage) {
+void clue2d_variant_shape(int variant, int& points, int& stage) {
   const Variant2D v = kVariants2D[variant >= 0 && variant < kNumVariants2D ? variant : 0];
-  items = v.items;
+  points = v.threads * v.items;
   stage = v.stage;
 }
 

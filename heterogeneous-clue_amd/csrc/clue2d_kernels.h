@@ -376,34 +376,38 @@ __global__ __launch_bounds__(NT) void k_segsort2d(const int* __restrict__ evOff,
   }
 }
 
-// The density and nearest-higher kernels give each block ITEMS * kThreads consecutive sorted points
-// of one event (ITEMS points per thread). In x-major tile order everything their search boxes can
-// reach lies in one contiguous range of the sorted array, whole columns from the lowest box's first
-// column to the highest box's last (reach_columns, computed from the block's first and last keys, so
-// staging starts without waiting for the points' own bin-table lookups). When that range has at
-// most STAGE points it is staged into LDS once and every candidate is read from there; otherwise
-// candidates are read from global memory. Either way the
-// candidates are visited in the serial order, so the results do not depend on ITEMS, STAGE or which
-// path ran; the plan autotuner picks (ITEMS, STAGE) per kernel among kVariants2D by timing.
-template <int ITEMS>
+// The density and nearest-higher kernels give each block of NT threads NT * ITEMS consecutive sorted
+// points of one event (ITEMS points per thread). In x-major tile order everything their search boxes
+// can reach lies in one contiguous range of the sorted array, whole columns from the lowest box's
+// first column to the highest box's last (reach_columns, computed from the block's first and last
+// keys, so staging starts without waiting for the points' own bin-table lookups). When that range
+// has at most STAGE points it is staged into LDS once and every candidate is read from there;
+// otherwise candidates are read from global memory. Either way the candidates are visited in the
+// serial order, so the results do not depend on NT, ITEMS, STAGE or which path ran; the plan
+// autotuner picks (NT, ITEMS, STAGE) per kernel among kVariants2D by timing. Larger blocks stage
+// fewer candidates per point (the reach beyond the block's own columns is shared by more points).
+template <int NT, int ITEMS>
 struct CompShape {
-  static constexpr int kHits = kThreads * ITEMS;         // points per compute block
+  static constexpr int kHits = NT * ITEMS;               // points per compute block
   static constexpr int kPerSort = kSortBlock / kHits;    // compute blocks per sort block
-  static_assert(kSortBlock % kHits == 0, "compute blocks tile each sort block");
+  static constexpr int kTiles = kHits / kThreads;        // 256-point reach tiles per compute block
+  static_assert(kSortBlock % kHits == 0 && kHits % kThreads == 0, "compute blocks tile each sort block");
 };
 constexpr int kMaxCols = 5;  // column runs kept in registers per point
 
-// Waves per SIMD the compiler must allow for the variants that hold one or two points per thread
-// with the 1024-point stage (their LDS admits 8 and 7): it keeps their registers to 64 and 72.
-template <int ITEMS, int STAGE>
+// Waves per SIMD the compiler must allow: one point per thread with the 1024-point stage (or any
+// larger block) admits 8 by LDS, two points per thread with it 7; registers are held to 64 and 72.
+template <int NT, int ITEMS, int STAGE>
 constexpr int kWavesPerEu2D = ITEMS == 1 ? 8 : (ITEMS == 2 && STAGE == 1024) ? 7 : 1;
 
 struct Variant2D {
-  int items;  // points per thread
-  int stage;  // staged points
+  int threads;  // threads per block
+  int items;    // points per thread
+  int stage;    // staged points
 };
-constexpr int kNumVariants2D = 4;
-constexpr Variant2D kVariants2D[kNumVariants2D] = {{2, 1024}, {1, 1024}, {2, 2048}, {4, 2048}};
+constexpr int kNumVariants2D = 6;
+constexpr Variant2D kVariants2D[kNumVariants2D] = {{256, 2, 1024}, {256, 1, 1024}, {256, 2, 2048},
+                                                   {256, 4, 2048}, {512, 1, 1536}, {1024, 1, 2048}};
 
 // A search box of the reference (LayerTilesSerial::searchBox) as column runs: run k covers column
 // xlo + k, rows ylo..yhi, which are the sorted positions [s[k], t[k]). Columns beyond kMaxCols
@@ -511,13 +515,13 @@ __global__ __launch_bounds__(kReachThreads) void k_reach2d(const BlockDesc* __re
   reachN[tile] = make_int4(loN, hiN, tloN, thiN);
 }
 
-// The reach of compute block `blk` of ITEMS tiles covering sorted points [b0, b1).
-template <int ITEMS>
+// The reach of compute block `blk` (Shape::kTiles tiles) covering sorted points [b0, b1).
+template <class Shape>
 __device__ __forceinline__ void compute_reach(const int4* __restrict__ reach, int blk, int b0, int b1, int& lo,
                                               int& hi, int& tlo, int& tn) {
-  constexpr int kPerSort = kSortBlock / (kThreads * ITEMS);
-  const int t0 = (blk / kPerSort) * kTilesPerSort + (blk % kPerSort) * ITEMS;
-  const int t1 = t0 + (b1 - b0 - 1) / kThreads;
+  const int t0 = (blk / Shape::kPerSort) * kTilesPerSort + (blk % Shape::kPerSort) * Shape::kTiles;
+  // a one-tile block reads one record, whose load then need not wait for the block's range
+  const int t1 = Shape::kTiles == 1 ? t0 : t0 + (b1 - b0 - 1) / kThreads;
   const int4 r0 = reach[t0], r1 = reach[t1];
   lo = r0.x;
   hi = r1.y;
@@ -530,7 +534,8 @@ __device__ __forceinline__ void compute_reach(const int4* __restrict__ reach, in
 // one 16-byte load per thread, and the starts of its layers (up to 101, one load per thread). It is
 // staged when it spans at most kTab16Stage entries (3 KB of LDS).
 constexpr int kTab16Stage = 1536;
-static_assert(kTab16Stage % 8 == 0 && kTab16Stage / 8 <= kThreads && k2dLayers + 1 <= kThreads, "one load per thread");
+static_assert(kTab16Stage % 8 == 0 && kTab16Stage / 8 <= kThreads && k2dLayers + 1 <= kThreads,
+              "one load per thread of the smallest block");
 struct TabWindow {
   int base, n, L0, nStarts;
   bool fits;
@@ -599,43 +604,44 @@ __device__ __forceinline__ float4 cand(const NhSrc& g, const float4* st, int q, 
     return g.at(q);
 }
 
-// stage_range for the nearest-higher candidates.
-template <int STAGE>
+// Copy of [lo, hi) (1 <= hi - lo <= STAGE points) into LDS for the nearest-higher candidates, all
+// loads of a thread in flight together (indices past the end re-read the last point, not stored).
+template <int NT, int STAGE>
 __device__ __forceinline__ void stage_range_nh(const NhSrc& g, float4* st, int lo, int hi) {
-  static_assert(STAGE % kThreads == 0, "staging splits evenly over the block");
-  constexpr int kPer = STAGE / kThreads;
+  static_assert(STAGE % NT == 0, "staging splits evenly over the block");
+  constexpr int kPer = STAGE / NT;
   const int n = hi - lo;
   float4 v[kPer];
   uint32_t r[kPer];
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
-    const int k = u * kThreads + (int)threadIdx.x;
+    const int k = u * NT + (int)threadIdx.x;
     const int q = lo + (k < n ? k : n - 1);
     v[u] = g.xyw[q];
     r[u] = g.rhoKey[q];
   }
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
-    const int k = u * kThreads + (int)threadIdx.x;
+    const int k = u * NT + (int)threadIdx.x;
     if (k < n) st[k] = make_float4(v[u].x, v[u].y, v[u].w, __int_as_float((int)r[u]));
   }
 }
 
-// stage_range for the density candidates: {x, y, 0.5f * w, index} (see DensityStaged).
-template <int STAGE>
+// The same for the density candidates: {x, y, 0.5f * w, index} (see DensityStaged).
+template <int NT, int STAGE>
 __device__ __forceinline__ void stage_range_half(const float4* __restrict__ g, float4* st, int lo, int hi) {
-  static_assert(STAGE % kThreads == 0, "staging splits evenly over the block");
-  constexpr int kPer = STAGE / kThreads;
+  static_assert(STAGE % NT == 0, "staging splits evenly over the block");
+  constexpr int kPer = STAGE / NT;
   const int n = hi - lo;
   float4 v[kPer];
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
-    const int k = u * kThreads + (int)threadIdx.x;
+    const int k = u * NT + (int)threadIdx.x;
     v[u] = g[lo + (k < n ? k : n - 1)];
   }
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
-    const int k = u * kThreads + (int)threadIdx.x;
+    const int k = u * NT + (int)threadIdx.x;
     if (k < n) st[k] = make_float4(v[u].x, v[u].y, 0.5f * v[u].z, v[u].w);
   }
 }
@@ -803,13 +809,13 @@ __device__ __forceinline__ float density_run(int p, float xi, float yi, int s, i
   return rho;
 }
 
-template <int ITEMS, int STAGE>
-__global__ __launch_bounds__(kThreads) CLUE_WAVES_PER_EU(kWavesPerEu2D<ITEMS, STAGE>) void k_density2d(const BlockDesc* __restrict__ blocks,
+template <int NT, int ITEMS, int STAGE>
+__global__ __launch_bounds__(NT) CLUE_WAVES_PER_EU(kWavesPerEu2D<NT, ITEMS, STAGE>) void k_density2d(const BlockDesc* __restrict__ blocks,
                                                         const uint32_t* __restrict__ keyS,
                                                         const float4* __restrict__ xyw, BinTables2D tabs,
                                                         const int4* __restrict__ reach, float dc, float dc2,
                                                         uint32_t* __restrict__ rhoKey, float* __restrict__ rhoOut) {
-  using Shape = CompShape<ITEMS>;
+  using Shape = CompShape<NT, ITEMS>;
   static_assert(STAGE <= kStageMax, "staged byte offsets take 16 bits");
   __shared__ float4 s_st[STAGE + 1];  // + the spare element a masked second candidate may read
   __shared__ __align__(16) uint16_t s_tab[kTab16Stage];
@@ -822,9 +828,9 @@ __global__ __launch_bounds__(kThreads) CLUE_WAVES_PER_EU(kWavesPerEu2D<ITEMS, ST
   if (b0 >= b1) return;  // uniform over the block
   const BinTable2D T = tabs.event(bd.event);
   int rlo, rhi, tlo, tn;
-  compute_reach<ITEMS>(reach, blk, b0, b1, rlo, rhi, tlo, tn);
+  compute_reach<Shape>(reach, blk, b0, b1, rlo, rhi, tlo, tn);
   const bool staged = rhi - rlo <= STAGE;
-  if (staged) stage_range_half<STAGE>(xyw, s_st, rlo, rhi);
+  if (staged) stage_range_half<NT, STAGE>(xyw, s_st, rlo, rhi);
   const TabWindow win = table_window(tlo, tn);
   const TableView tab{T, s_tab, s_layerStart, win.base, win.L0, win.fits};
   if (tab.staged) stage_table(T, win, s_tab, s_layerStart);
@@ -832,7 +838,7 @@ __global__ __launch_bounds__(kThreads) CLUE_WAVES_PER_EU(kWavesPerEu2D<ITEMS, ST
   uint32_t key[ITEMS];
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
-    const int p = b0 + r * kThreads + (int)threadIdx.x;
+    const int p = b0 + r * NT + (int)threadIdx.x;
     me[r] = xyw[p < b1 ? p : b0];
     key[r] = keyS[p < b1 ? p : b0];
   }
@@ -842,7 +848,7 @@ __global__ __launch_bounds__(kThreads) CLUE_WAVES_PER_EU(kWavesPerEu2D<ITEMS, ST
   for (int r = 0; r < ITEMS; ++r) make_runs(tab, key[r], me[r].x, me[r].y, dc, runs[r]);
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
-    const int p = b0 + r * kThreads + (int)threadIdx.x;
+    const int p = b0 + r * NT + (int)threadIdx.x;
     if (p < b1) {
       const float xi = me[r].x, yi = me[r].y;
       float rho = 0.f;
@@ -1015,8 +1021,8 @@ __device__ __forceinline__ int nearest_higher_of(uint64_t qkey, float xi, float 
 
 // delta / nearestHigher plus the seed/outlier decision of kernel_findAndAssign_clusters
 // (CLUEAlgoKernels.h:108-137).
-template <int ITEMS, int STAGE>
-__global__ __launch_bounds__(kThreads) CLUE_WAVES_PER_EU(kWavesPerEu2D<ITEMS, STAGE>) void k_nh2d(const BlockDesc* __restrict__ blocks,
+template <int NT, int ITEMS, int STAGE>
+__global__ __launch_bounds__(NT) CLUE_WAVES_PER_EU(kWavesPerEu2D<NT, ITEMS, STAGE>) void k_nh2d(const BlockDesc* __restrict__ blocks,
                                                    const uint32_t* __restrict__ keyS,
                                                    const float4* __restrict__ xyw, const uint32_t* __restrict__ rhoKey,
                                                    BinTables2D tabs,
@@ -1025,7 +1031,7 @@ __global__ __launch_bounds__(kThreads) CLUE_WAVES_PER_EU(kWavesPerEu2D<ITEMS, ST
                                                    float odf, float dm2, float dm2up, float* __restrict__ deltaOut,
                                                    int32_t* __restrict__ nhOut, int32_t* __restrict__ isSeedOut,
                                                    int* __restrict__ parent) {
-  using Shape = CompShape<ITEMS>;
+  using Shape = CompShape<NT, ITEMS>;
   static_assert(STAGE <= kStageMax, "staged byte offsets take 16 bits");
   __shared__ float4 s_st[STAGE + 1];  // + the spare element a masked second candidate may read
   __shared__ __align__(16) uint16_t s_tab[kTab16Stage];
@@ -1041,9 +1047,9 @@ __global__ __launch_bounds__(kThreads) CLUE_WAVES_PER_EU(kWavesPerEu2D<ITEMS, ST
   const float dm = odf * dc;
   const NhSrc src{xyw, rhoKey};
   int rlo, rhi, tlo, tn;
-  compute_reach<ITEMS>(reach, blk, b0, b1, rlo, rhi, tlo, tn);
+  compute_reach<Shape>(reach, blk, b0, b1, rlo, rhi, tlo, tn);
   const bool staged = rhi - rlo <= STAGE;
-  if (staged) stage_range_nh<STAGE>(src, s_st, rlo, rhi);
+  if (staged) stage_range_nh<NT, STAGE>(src, s_st, rlo, rhi);
   const TabWindow win = table_window(tlo, tn);
   const TableView tab{T, s_tab, s_layerStart, win.base, win.L0, win.fits};
   if (tab.staged) stage_table(T, win, s_tab, s_layerStart);
@@ -1051,7 +1057,7 @@ __global__ __launch_bounds__(kThreads) CLUE_WAVES_PER_EU(kWavesPerEu2D<ITEMS, ST
   uint32_t key[ITEMS];
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
-    const int p = b0 + r * kThreads + (int)threadIdx.x;
+    const int p = b0 + r * NT + (int)threadIdx.x;
     me[r] = src.at(p < b1 ? p : b0);
     key[r] = keyS[p < b1 ? p : b0];
   }
@@ -1061,7 +1067,7 @@ __global__ __launch_bounds__(kThreads) CLUE_WAVES_PER_EU(kWavesPerEu2D<ITEMS, ST
   for (int r = 0; r < ITEMS; ++r) make_runs(tab, key[r], me[r].x, me[r].y, dm, runs[r]);
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
-    const int p = b0 + r * kThreads + (int)threadIdx.x;
+    const int p = b0 + r * NT + (int)threadIdx.x;
     if (p < b1) {
       const int i = __float_as_int(me[r].z);
       const uint32_t rk = (uint32_t)__float_as_int(me[r].w);

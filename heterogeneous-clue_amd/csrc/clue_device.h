@@ -104,8 +104,12 @@ __device__ __forceinline__ int xcd_block(int b, int G) {
 // ---------------------------------------------------------------- wave / block primitives
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
-// Mask of the lanes of this wave whose `digit` (< 128) equals this lane's digit.
+// Mask of the lanes of this wave whose `digit` (< 128) equals this lane's digit (every lane of the
+// wave active). When all lanes hold lane 0's digit, as for keys that arrive grouped, one ballot
+// answers; otherwise each of the digit's seven bits is matched with a ballot.
 __device__ __forceinline__ uint64_t match_digit7(int digit) {
+  const uint64_t same = __ballot(digit == __shfl(digit, 0, kWave));
+  if (same == ~0ull) return same;
   uint64_t m = ~0ull;
 #pragma unroll
   for (int b = 0; b < 7; ++b) {

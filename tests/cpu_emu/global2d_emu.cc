@@ -15,14 +15,18 @@ using namespace emu_host;
 template <int V, class F>
 void with_variant(F&& f) {
   constexpr Variant2D v = kVariants2D[V];
-  f(std::integral_constant<int, v.items>{}, std::integral_constant<int, v.stage>{});
+  f(std::integral_constant<int, v.threads>{}, std::integral_constant<int, v.items>{},
+    std::integral_constant<int, v.stage>{});
 }
 template <class F>
 void dispatch(int variant, F&& f) {
+  static_assert(kNumVariants2D == 6, "dispatch covers every variant");
   switch (variant) {
     case 1: with_variant<1>(f); break;
     case 2: with_variant<2>(f); break;
     case 3: with_variant<3>(f); break;
+    case 4: with_variant<4>(f); break;
+    case 5: with_variant<5>(f); break;
     default: with_variant<0>(f); break;
   }
 }
@@ -82,17 +86,17 @@ int main(int argc, char** argv) {
       k_reach2d(blocks.data(), nTiles, keyS.data(), tabs, dc, odf, reachD.data(), reachN.data());
     });
     const int dv = argc > 3 ? atoi(argv[3]) : 0, nv = argc > 4 ? atoi(argv[4]) : 0;
-    dispatch(dv, [&](auto I, auto S) {
-      emu::launch(CompShape<I>::kPerSort * nb, kThreads, [&] {
-        k_density2d<I, S>(blocks.data(), keyS.data(), xyw.data(), tabs, reachD.data(), dc, dc2,
-                          rhoKey.data(), rho.data());
+    dispatch(dv, [&](auto T, auto I, auto S) {
+      emu::launch(CompShape<T, I>::kPerSort * nb, T, [&] {
+        k_density2d<T, I, S>(blocks.data(), keyS.data(), xyw.data(), tabs, reachD.data(), dc, dc2,
+                             rhoKey.data(), rho.data());
       });
     });
     int* parent = reinterpret_cast<int*>(lg.data());
-    dispatch(nv, [&](auto I, auto S) {
-      emu::launch(CompShape<I>::kPerSort * nb, kThreads, [&] {
-        k_nh2d<I, S>(blocks.data(), keyS.data(), xyw.data(), rhoKey.data(), tabs, reachN.data(),
-                     evOff.data(), dc, rhoc, odf, dm2, dm2up, delta.data(), nh.data(), isSeed.data(), parent);
+    dispatch(nv, [&](auto T, auto I, auto S) {
+      emu::launch(CompShape<T, I>::kPerSort * nb, T, [&] {
+        k_nh2d<T, I, S>(blocks.data(), keyS.data(), xyw.data(), rhoKey.data(), tabs, reachN.data(),
+                        evOff.data(), dc, rhoc, odf, dm2, dm2up, delta.data(), nh.data(), isSeed.data(), parent);
       });
     });
     SeedRankHost seeds(N);

@@ -167,7 +167,7 @@ def test_emulated_global_wide_layer(emulator, tmp_path, oracle_lib):
     helpers.assert_bit_exact(got, want, helpers.OUT2D, "emulated wide layer")
 
 
-@pytest.mark.parametrize("variants", [(1, 1), (2, 3), (3, 2)])
+@pytest.mark.parametrize("variants", [(1, 1), (2, 3), (3, 2), (4, 5), (5, 4)])
 def test_emulated_global_variants(emulator, tmp_path, variants):
     """Every (points per workgroup, LDS staging) variant of density and nearest higher gives the same bits."""
     ins, want, extra = helpers.load_case("clue2d", "bin_edges")
