@@ -6,16 +6,14 @@
 // of its batch), kept resident. The arenas share the context's HBM budget: each slot may hold up to
 // budget / slots bytes (without a budget, what its largest batch of max_events events and
 // max_points points per dimension needs), and an arena grows to that share only when a batch needs
-// it. Whenever a slot is free and requests are queued, the worker admits queued requests in arrival
+// it. Whenever a slot is free and requests are queued, the worker admits the queue head in arrival
 // order while the batch's device bytes fit the slot's share (and its event and point counts fit the
-// staging), at most its part of the queue when several slots are free, packs the batch into the
-// slot's pinned staging, and enqueues copy-in on the server's upload stream, the batched kernels on
-// the slot's stream and copy-out on the server's download stream, chained by events. Requests that
-// arrive meanwhile wait for the next free slot, so with two or more slots a new batch starts while
-// earlier ones still run: one batch's upload overlaps another's kernels and a third's download,
-// while the transfers of one direction keep admission order. When a slot's completion event fires
-// the worker copies each result to its caller's buffers, reads the event's device error flags, and
-// timestamps it; latency = completion time - submission time.
+// staging), packs the batch into the slot's pinned staging, and enqueues copy-in, the batched
+// kernels and copy-out on the slot's stream. Requests that arrive meanwhile wait for the next free
+// slot, so with two or more slots a new batch starts while earlier ones still run and their work
+// overlaps on the device. When a slot's completion event fires the worker copies each result to its
+// caller's buffers, reads the event's device error flags, and timestamps it; latency = completion
+// time - submission time.
 //
 // Every event's arithmetic depends only on its own points (blocks never straddle events), so a
 // result is bit-identical whether the event ran alone or in any mix.
@@ -144,10 +142,8 @@ bool is_pinned(const void* p) {
 constexpr int kIn2 = 4, kOut2 = 5, kIn3 = 8, kOut3 = 7;
 
 struct Slot {
-  hipStream_t stream = nullptr;  // the batch's kernels
-  hipEvent_t inDone = nullptr;   // its inputs are on the device (upload stream)
-  hipEvent_t kernDone = nullptr;  // its kernels have run (slot stream)
-  hipEvent_t done = nullptr;     // its outputs are in host memory (download stream)
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
   Workspace arena;        // device: the batch's SoA, workspaces and error words (batch_layout)
   char* stage = nullptr;  // pinned host staging
   size_t stageBytes = 0;
@@ -166,7 +162,6 @@ struct clue_server {
   int capEvents = 1;
   size_t share = 0;            // device bytes one slot's batch may use
   std::vector<Slot> slots;
-  hipStream_t up = nullptr, down = nullptr;  // every batch's host-to-device and device-to-host copies
   std::mutex m;
   std::condition_variable cvWork, cvDone;
   std::deque<Req*> queue;
@@ -254,7 +249,7 @@ int server_fail(clue_server* s, int code, const std::string& msg) {
 // Inputs of a batch into the slot's device SoA (field k at dev + k*stride, event e at offs[e]):
 // pinned caller buffers by DMA directly; pageable ones through the pinned staging (parallel host
 // copies) and one DMA per contiguous staged run.
-hipError_t copy_in(clue_server* s, hipStream_t st, const std::vector<Req*>& reqs, const std::vector<int64_t>& offs,
+hipError_t copy_in(clue_server* s, Slot& sl, const std::vector<Req*>& reqs, const std::vector<int64_t>& offs,
                    int nf, char* stage, size_t sstride, char* dev, size_t stride) {
   bool staged = false;
   for (size_t e = 0; e < reqs.size(); ++e)
@@ -269,7 +264,7 @@ hipError_t copy_in(clue_server* s, hipStream_t st, const std::vector<Req*>& reqs
     while (e < reqs.size()) {
       if (reqs[e]->pinnedIn) {
         hipError_t r = hipMemcpyAsync(dev + k * stride + (size_t)offs[e] * 4, reqs[e]->in[k], (size_t)reqs[e]->n * 4,
-                                      hipMemcpyHostToDevice, st);
+                                      hipMemcpyHostToDevice, sl.stream);
         if (r != hipSuccess) return r;
         ++e;
         continue;
@@ -277,7 +272,8 @@ hipError_t copy_in(clue_server* s, hipStream_t st, const std::vector<Req*>& reqs
       size_t f = e;
       while (f < reqs.size() && !reqs[f]->pinnedIn) ++f;  // a run of staged events
       const size_t a = (size_t)offs[e] * 4, b = (size_t)offs[f] * 4;
-      hipError_t r = hipMemcpyAsync(dev + k * stride + a, stage + k * sstride + a, b - a, hipMemcpyHostToDevice, st);
+      hipError_t r = hipMemcpyAsync(dev + k * stride + a, stage + k * sstride + a, b - a, hipMemcpyHostToDevice,
+                                    sl.stream);
       if (r != hipSuccess) return r;
       e = f;
     }
@@ -287,14 +283,14 @@ hipError_t copy_in(clue_server* s, hipStream_t st, const std::vector<Req*>& reqs
 
 // Outputs of a batch: into pinned caller buffers directly, otherwise into the staging (copied to
 // the caller by finish_slot).
-hipError_t copy_out(hipStream_t st, const std::vector<Req*>& reqs, const std::vector<int64_t>& offs, int nf, char* stage,
+hipError_t copy_out(Slot& sl, const std::vector<Req*>& reqs, const std::vector<int64_t>& offs, int nf, char* stage,
                     size_t sstride, char* dev, size_t stride) {
   for (int k = 0; k < nf; ++k) {
     size_t e = 0;
     while (e < reqs.size()) {
       if (reqs[e]->pinnedOut) {
         hipError_t r = hipMemcpyAsync(reqs[e]->out[k], dev + k * stride + (size_t)offs[e] * 4, (size_t)reqs[e]->n * 4,
-                                      hipMemcpyDeviceToHost, st);
+                                      hipMemcpyDeviceToHost, sl.stream);
         if (r != hipSuccess) return r;
         ++e;
         continue;
@@ -302,7 +298,8 @@ hipError_t copy_out(hipStream_t st, const std::vector<Req*>& reqs, const std::ve
       size_t f = e;
       while (f < reqs.size() && !reqs[f]->pinnedOut) ++f;
       const size_t a = (size_t)offs[e] * 4, b = (size_t)offs[f] * 4;
-      hipError_t r = hipMemcpyAsync(stage + k * sstride + a, dev + k * stride + a, b - a, hipMemcpyDeviceToHost, st);
+      hipError_t r = hipMemcpyAsync(stage + k * sstride + a, dev + k * stride + a, b - a, hipMemcpyDeviceToHost,
+                                    sl.stream);
       if (r != hipSuccess) return r;
       e = f;
     }
@@ -344,77 +341,68 @@ int launch_slot(clue_server* s, Slot& sl) {
   char* dev = reinterpret_cast<char*>(sl.arena.base);
   int* err2 = reinterpret_cast<int*>(dev + B.err2);
   int* err3 = reinterpret_cast<int*>(dev + B.err3);
-  // The batch runs as three stages: its inputs and block tables on the server's upload stream, its
-  // kernels on the slot's stream, its outputs on the server's download stream, each stage waiting
-  // for the one before through an event. The transfers of each direction go one batch after the
-  // other in admission order, so a batch never shares PCIe bandwidth with a later one in the same
-  // direction, while the uploads, kernels and downloads of different batches overlap.
-  Clue2DBuffers b2;
-  Clue2DIO io2{};
-  char* dout2 = nullptr;
+  fill_i32(err2, 0, (int64_t)(B.err3 - B.err2) / 4, sl.stream);
+  fill_i32(err3, 0, (int64_t)(B.in2 - B.err3) / 4, sl.stream);
+  // ---- 2D
   if (E2 > 0) {
     const int64_t N = sl.offs2[E2];
     char* din = dev + B.in2;
-    HIP_TRY(c, copy_in(s, s->up, sl.reqs2, sl.offs2, kIn2, st + L.in2, sf2, din, B.f2));
-    clue2d_carve(dev + B.ws2, N, E2, (int)nb2, b2);
-    b2.err = err2;
-    b2.errStride = 1;
+    HIP_TRY(c, copy_in(s, sl, sl.reqs2, sl.offs2, kIn2, st + L.in2, sf2, din, B.f2));
+    Clue2DBuffers b;
+    clue2d_carve(dev + B.ws2, N, E2, (int)nb2, b);
+    b.err = err2;
+    b.errStride = 1;
     const std::vector<BlockDesc> blocks = make_blocks(sl.offs2.data(), E2);
     std::memcpy(st + L.blk2, blocks.data(), blocks.size() * sizeof(BlockDesc));
     int* off32 = reinterpret_cast<int*>(st + L.off2);
     for (int e = 0; e <= E2; ++e) off32[e] = (int)sl.offs2[e];
-    HIP_TRY(c, hipMemcpyAsync(const_cast<BlockDesc*>(b2.blocks), st + L.blk2, blocks.size() * sizeof(BlockDesc),
-                              hipMemcpyHostToDevice, s->up));
-    HIP_TRY(c, hipMemcpyAsync(const_cast<int*>(b2.evOff), off32, (size_t)(E2 + 1) * 4, hipMemcpyHostToDevice, s->up));
-    dout2 = din + kIn2 * B.f2;
+    HIP_TRY(c, hipMemcpyAsync(const_cast<BlockDesc*>(b.blocks), st + L.blk2, blocks.size() * sizeof(BlockDesc),
+                              hipMemcpyHostToDevice, sl.stream));
+    HIP_TRY(c, hipMemcpyAsync(const_cast<int*>(b.evOff), off32, (size_t)(E2 + 1) * 4, hipMemcpyHostToDevice,
+                              sl.stream));
+    char* dout = din + kIn2 * B.f2;
     const size_t f = B.f2;
-    io2 = Clue2DIO{reinterpret_cast<float*>(din), reinterpret_cast<float*>(din + f),
-                   reinterpret_cast<int32_t*>(din + 2 * f), reinterpret_cast<float*>(din + 3 * f),
-                   reinterpret_cast<float*>(dout2), reinterpret_cast<float*>(dout2 + f),
-                   reinterpret_cast<int32_t*>(dout2 + 2 * f), reinterpret_cast<int32_t*>(dout2 + 3 * f),
-                   reinterpret_cast<int32_t*>(dout2 + 4 * f)};
+    Clue2DIO io{reinterpret_cast<float*>(din), reinterpret_cast<float*>(din + f),
+                reinterpret_cast<int32_t*>(din + 2 * f), reinterpret_cast<float*>(din + 3 * f),
+                reinterpret_cast<float*>(dout), reinterpret_cast<float*>(dout + f),
+                reinterpret_cast<int32_t*>(dout + 2 * f), reinterpret_cast<int32_t*>(dout + 3 * f),
+                reinterpret_cast<int32_t*>(dout + 4 * f)};
+    clue2d_enqueue(b, io, s->p2, sl.stream);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, copy_out(sl, sl.reqs2, sl.offs2, kOut2, st + L.out2, sf2, dout, B.f2));
   }
-  Clue3DBuffers b3;
-  Clue3DIO io3{};
-  char* dout3 = nullptr;
+  // ---- 3D
   if (E3 > 0) {
     const int64_t N = sl.offs3[E3];
     char* din = dev + B.in3;
-    HIP_TRY(c, copy_in(s, s->up, sl.reqs3, sl.offs3, kIn3, st + L.in3, sf3, din, B.f3));
-    clue3d_carve(dev + B.ws3, N, E3, (int)nb3, b3);
-    b3.err = err3;
-    b3.errStride = 1;
+    HIP_TRY(c, copy_in(s, sl, sl.reqs3, sl.offs3, kIn3, st + L.in3, sf3, din, B.f3));
+    Clue3DBuffers b;
+    clue3d_carve(dev + B.ws3, N, E3, (int)nb3, b);
+    b.err = err3;
+    b.errStride = 1;
     const std::vector<BlockDesc> blocks = make_blocks(sl.offs3.data(), E3);
     std::memcpy(st + L.blk3, blocks.data(), blocks.size() * sizeof(BlockDesc));
     int* off32 = reinterpret_cast<int*>(st + L.off3);
     for (int e = 0; e <= E3; ++e) off32[e] = (int)sl.offs3[e];
-    HIP_TRY(c, hipMemcpyAsync(const_cast<BlockDesc*>(b3.blocks), st + L.blk3, blocks.size() * sizeof(BlockDesc),
-                              hipMemcpyHostToDevice, s->up));
-    HIP_TRY(c, hipMemcpyAsync(const_cast<int*>(b3.evOff), off32, (size_t)(E3 + 1) * 4, hipMemcpyHostToDevice, s->up));
-    dout3 = din + kIn3 * B.f3;
+    HIP_TRY(c, hipMemcpyAsync(const_cast<BlockDesc*>(b.blocks), st + L.blk3, blocks.size() * sizeof(BlockDesc),
+                              hipMemcpyHostToDevice, sl.stream));
+    HIP_TRY(c, hipMemcpyAsync(const_cast<int*>(b.evOff), off32, (size_t)(E3 + 1) * 4, hipMemcpyHostToDevice,
+                              sl.stream));
+    char* dout = din + kIn3 * B.f3;
     auto F = [&](char* base, int k) { return reinterpret_cast<float*>(base + k * B.f3); };
     auto I = [&](char* base, int k) { return reinterpret_cast<int32_t*>(base + k * B.f3); };
-    io3 = Clue3DIO{F(din, 0), F(din, 1), F(din, 2), F(din, 3), F(din, 4), F(din, 5), I(din, 6), I(din, 7),
-                   F(dout3, 0), F(dout3, 1), I(dout3, 2), I(dout3, 3), I(dout3, 4), I(dout3, 5), I(dout3, 6)};
+    Clue3DIO io{F(din, 0), F(din, 1), F(din, 2), F(din, 3), F(din, 4), F(din, 5), I(din, 6), I(din, 7),
+                F(dout, 0), F(dout, 1), I(dout, 2), I(dout, 3), I(dout, 4), I(dout, 5), I(dout, 6)};
+    clue3d_enqueue(b, io, s->p3, sl.stream);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, copy_out(sl, sl.reqs3, sl.offs3, kOut3, st + L.out3, sf3, dout, B.f3));
   }
-  HIP_TRY(c, hipEventRecord(sl.inDone, s->up));
-  // kernels
-  HIP_TRY(c, hipStreamWaitEvent(sl.stream, sl.inDone, 0));
-  fill_i32(err2, 0, (int64_t)(B.err3 - B.err2) / 4, sl.stream);
-  fill_i32(err3, 0, (int64_t)(B.in2 - B.err3) / 4, sl.stream);
-  if (E2 > 0) clue2d_enqueue(b2, io2, s->p2, sl.stream);
-  if (E3 > 0) clue3d_enqueue(b3, io3, s->p3, sl.stream);
-  HIP_TRY(c, hipGetLastError());
-  HIP_TRY(c, hipEventRecord(sl.kernDone, sl.stream));
-  // outputs, and the events' error words with them
-  HIP_TRY(c, hipStreamWaitEvent(s->down, sl.kernDone, 0));
-  if (E2 > 0) HIP_TRY(c, copy_out(s->down, sl.reqs2, sl.offs2, kOut2, st + L.out2, sf2, dout2, B.f2));
-  if (E3 > 0) HIP_TRY(c, copy_out(s->down, sl.reqs3, sl.offs3, kOut3, st + L.out3, sf3, dout3, B.f3));
+  // the events' error words come back with the results
   int* herr = reinterpret_cast<int*>(st + L.err);
-  if (E2 > 0) HIP_TRY(c, hipMemcpyAsync(herr, err2, (size_t)E2 * 4, hipMemcpyDeviceToHost, s->down));
+  if (E2 > 0) HIP_TRY(c, hipMemcpyAsync(herr, err2, (size_t)E2 * 4, hipMemcpyDeviceToHost, sl.stream));
   if (E3 > 0)
-    HIP_TRY(c, hipMemcpyAsync(herr + s->capEvents, err3, (size_t)E3 * 4, hipMemcpyDeviceToHost, s->down));
-  HIP_TRY(c, hipEventRecord(sl.done, s->down));
+    HIP_TRY(c, hipMemcpyAsync(herr + s->capEvents, err3, (size_t)E3 * 4, hipMemcpyDeviceToHost, sl.stream));
+  HIP_TRY(c, hipEventRecord(sl.done, sl.stream));
   return CLUE_OK;
 }
 
@@ -488,21 +476,15 @@ void worker_loop(clue_server* s) {
     // fit the slot's share of the HBM budget and its points and events fit the staging
     Slot* free = nullptr;
     bool anyBusy = false;
-    int nFree = 0;
     for (auto& sl : s->slots) {
       anyBusy = anyBusy || sl.busy;
-      nFree += sl.busy ? 0 : 1;
       if (!sl.busy && !free) free = &sl;
     }
     if (free) {
       std::unique_lock<std::mutex> lk(s->m);
       if (s->stop && s->queue.empty() && !anyBusy) break;  // drained: every batch has been finished
-      // The queued events are spread over the free slots (this slot takes its share, rounded up):
-      // under light load every event gets a slot of its own and its copies and kernels overlap those
-      // of the others; under heavy load one slot frees at a time and takes all it can hold.
-      const int64_t take = ((int64_t)s->queue.size() + nFree - 1) / nFree;
       int64_t used2 = 0, used3 = 0, nb2 = 0, nb3 = 0;
-      while (!s->queue.empty() && (int64_t)(free->reqs2.size() + free->reqs3.size()) < take) {
+      while (!s->queue.empty()) {
         Req* r = s->queue.front();
         const bool d2 = r->dim == 2;
         std::vector<Req*>& dst = d2 ? free->reqs2 : free->reqs3;
@@ -531,9 +513,7 @@ void worker_loop(clue_server* s) {
         const int rc = launch_slot(s, *free);
         if (rc != CLUE_OK) {
           server_fail(s, rc, s->ctx->err);
-          (void)hipStreamSynchronize(s->up);
           (void)hipStreamSynchronize(free->stream);
-          (void)hipStreamSynchronize(s->down);
           finish_slot(s, *free, rc);
         }
         continue;
@@ -630,13 +610,8 @@ clue_server* clue_server_create(clue_ctx* c, const clue2d_params* p2, const clue
   }
   const size_t first = full.total < s->share ? full.total : s->share;
   bool ok = s->share > 0;
-  ok = ok && hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking) == hipSuccess;
-  ok = ok && hipStreamCreateWithFlags(&s->down, hipStreamNonBlocking) == hipSuccess;
   for (auto& sl : s->slots) {
-    if (!ok) break;
     ok = ok && hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) == hipSuccess;
-    ok = ok && hipEventCreateWithFlags(&sl.inDone, hipEventDisableTiming) == hipSuccess;
-    ok = ok && hipEventCreateWithFlags(&sl.kernDone, hipEventDisableTiming) == hipSuccess;
     ok = ok && hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess;
     ok = ok && ensure(c, sl.arena, first) == CLUE_OK;
     if (ok) {
@@ -652,12 +627,8 @@ clue_server* clue_server_create(clue_ctx* c, const clue2d_params* p2, const clue
       if (sl.stage) (void)hipHostFree(sl.stage);
       if (sl.arena.base) dev_free(c, sl.arena.base);
       if (sl.done) (void)hipEventDestroy(sl.done);
-      if (sl.kernDone) (void)hipEventDestroy(sl.kernDone);
-      if (sl.inDone) (void)hipEventDestroy(sl.inDone);
       if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
-    if (s->up) (void)hipStreamDestroy(s->up);
-    if (s->down) (void)hipStreamDestroy(s->down);
     if (c->err.empty()) set_err(c, CLUE_ERR_HIP, "server resources could not be created");
     return nullptr;
   }
@@ -675,20 +646,13 @@ void clue_server_destroy(clue_server* s) {
   }
   s->cvWork.notify_all();
   if (s->worker.joinable()) s->worker.join();
-  for (auto& sl : s->slots)
-    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-  if (s->up) (void)hipStreamSynchronize(s->up);
-  if (s->down) (void)hipStreamSynchronize(s->down);
   for (auto& sl : s->slots) {
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
     if (sl.stage) (void)hipHostFree(sl.stage);
     if (sl.arena.base) dev_free(s->ctx, sl.arena.base);
     if (sl.done) (void)hipEventDestroy(sl.done);
-    if (sl.kernDone) (void)hipEventDestroy(sl.kernDone);
-    if (sl.inDone) (void)hipEventDestroy(sl.inDone);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
   }
-  if (s->up) (void)hipStreamDestroy(s->up);
-  if (s->down) (void)hipStreamDestroy(s->down);
   delete s;
 }
 
