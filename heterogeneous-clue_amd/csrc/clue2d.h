@@ -31,7 +31,7 @@ struct Clue2DBuffers {
   int nBlocks = 0;
   const BlockDesc* blocks = nullptr;  // device
   const int* evOff = nullptr;         // device, E + 1 (int32 offsets)
-  uint32_t* keyS = nullptr;  // tile order: bin keys
+  uint32_t* keyS = nullptr;  // tile order: bin keys (written at each 256-point tile's first and last position)
   int* scr = nullptr;        // slots of layer segments too large for LDS (k_segsort2d)
   float4* lg = nullptr;      // layer order: {x, y, weight, point index as int bits}; later the parent array
   int* segB = nullptr;       // per event, the first position of each layer (E * (k2dLayers + 1))
@@ -70,7 +70,8 @@ constexpr int64_t clue2d_table_stride() { return ((int64_t)k2dKeys + 1 + 63) / 6
 void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DParams& p, hipStream_t s);
 // The density (which = 0) or nearest-higher (which = 1) kernel alone with the given variant, on a
 // batch clue2d_enqueue has run on: both read only what earlier steps left and rewrite their own
-// outputs with the same values, so the autotuner can time them in place.
+// outputs with the same values (the seed bits are OR-ed in again), so the autotuner can time them
+// in place.
 void clue2d_enqueue_step(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DParams& p, int which, int variant,
                          hipStream_t s);
 constexpr int kClue2DVariants = 6;

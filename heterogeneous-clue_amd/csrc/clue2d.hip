@@ -92,6 +92,7 @@ static_assert(kNumVariants2D == 6, "launch_density2d / launch_nh2d dispatch six 
 
 void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DParams& prm, hipStream_t s) {
   if (b.N <= 0 || b.nBlocks <= 0) return;
+  clear_seed_words(b.seeds, b.N, s);  // the nearest-higher kernel sets one bit per seed
   const int nHist = kRadix * b.nBlocks;
   // tile order: a stable pass by layer into {x, y, w, index} records, then each (event, layer) sorted
   // by bin in its own workgroup, which also writes that layer's bin-table entries
@@ -113,27 +114,26 @@ void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DPara
   const int nTiles = b.nBlocks * kTilesPerSort;
   k_reach2d<<<(nTiles + kReachThreads - 1) / kReachThreads, kReachThreads, 0, s>>>(
       b.blocks, nTiles, keyS, tabs, prm.dc, prm.outlierDeltaFactor, b.reachD, b.reachN);
-  launch_density2d(b.densityVariant, b.nBlocks, s, b.blocks, keyS, b.xyw, tabs, (const int4*)b.reachD, prm.dc,
+  launch_density2d(b.densityVariant, b.nBlocks, s, b.blocks, (const float4*)b.xyw, tabs, (const int4*)b.reachD, prm.dc,
                    prm.dc2, b.rhoKey, io.rho);
   int* parent = reinterpret_cast<int*>(b.lg);
-  launch_nh2d(b.nhVariant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.rhoKey, tabs, (const int4*)b.reachN, b.evOff,
+  launch_nh2d(b.nhVariant, b.nBlocks, s, b.blocks, (const float4*)b.xyw, (const uint32_t*)b.rhoKey, tabs, (const int4*)b.reachN, b.evOff,
               prm.dc, prm.rhoc, prm.outlierDeltaFactor, prm.dm2, prm.dm2up, io.delta, io.nearestHigher, io.isSeed,
-              parent);
-  launch_labels(b.blocks, b.nBlocks, parent, io.isSeed, b.N, b.evOff, io.clusterIndex, b.seeds, b.part, s);
+              parent, b.seeds.word);
+  launch_labels(b.blocks, b.nBlocks, parent, nullptr, b.N, b.evOff, io.clusterIndex, b.seeds, b.part, s);
 }
 
 void clue2d_enqueue_step(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DParams& prm, int which, int variant,
                          hipStream_t s) {
   if (b.N <= 0 || b.nBlocks <= 0) return;
-  const uint32_t* keyS = b.keyS;
   const BinTables2D tabs{b.table16, b.table, b.segB, b.tableStride};
   int* parent = reinterpret_cast<int*>(b.lg);
   if (which == 0)
-    launch_density2d(variant, b.nBlocks, s, b.blocks, keyS, b.xyw, tabs, (const int4*)b.reachD,
+    launch_density2d(variant, b.nBlocks, s, b.blocks, (const float4*)b.xyw, tabs, (const int4*)b.reachD,
                      prm.dc, prm.dc2, b.rhoKey, io.rho);
   else
-    launch_nh2d(variant, b.nBlocks, s, b.blocks, keyS, b.xyw, b.rhoKey, tabs, (const int4*)b.reachN, b.evOff, prm.dc, prm.rhoc, prm.outlierDeltaFactor, prm.dm2, prm.dm2up, io.delta,
-                io.nearestHigher, io.isSeed, parent);
+    launch_nh2d(variant, b.nBlocks, s, b.blocks, (const float4*)b.xyw, (const uint32_t*)b.rhoKey, tabs, (const int4*)b.reachN, b.evOff, prm.dc, prm.rhoc, prm.outlierDeltaFactor, prm.dm2, prm.dm2up, io.delta,
+                io.nearestHigher, io.isSeed, parent, b.seeds.word);
 }
 
 void clue2d_variant_shape(int variant, int& points, int& stage) {

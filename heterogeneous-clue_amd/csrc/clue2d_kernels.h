@@ -30,8 +30,9 @@ namespace g2d {
 //      digit), whose output is each point as {x, y, w, index bits}, grouped by (event, layer) and in
 //      index order inside a layer (k_layer_starts2d then lists where each layer begins);
 //  (2) k_segsort2d: one workgroup per (event, layer) orders that layer's points by bin, keeping index
-//      order inside a bin, writes the sorted keys and {x, y, w, index} records, and writes the
-//      layer's 10^4 entries of the event's bin table from its own bin counts.
+//      order inside a bin, writes the sorted {x, y, w, index} records (and the bin keys of each
+//      256-point tile's first and last position), and writes the layer's 10^4 entries of the
+//      event's bin table from its own bin counts.
 
 // Layer of point i of event ev, the digit of step (1); an out-of-range layer sets kErrLayerRange in
 // err[ev * errStride] (errStride 0: one flag word for the whole batch) and is clamped.
@@ -216,8 +217,8 @@ __global__ __launch_bounds__(NT) void k_segsort2d(const int* __restrict__ evOff,
   __shared__ uint16_t s_slot[kPts];
   __shared__ int s_tmp[NT / kWave];
   const int e = (int)blockIdx.x / k2dLayers, L = (int)blockIdx.x % k2dLayers;
-  const int evEnd = evOff[e + 1];
-  if (evOff[e] == evEnd) return;  // an empty event: nothing reads its table (uniform over the block)
+  const int evBegin = evOff[e], evEnd = evOff[e + 1];
+  if (evBegin == evEnd) return;  // an empty event: nothing reads its table (uniform over the block)
   const int start = segB[e * (k2dLayers + 1) + L];
   const int n = segB[e * (k2dLayers + 1) + L + 1] - start;
   if (n < LO || (LO > 0 && n == LO) || (kLds && n > CAP)) return;  // another class (uniform)
@@ -370,7 +371,11 @@ __global__ __launch_bounds__(NT) void k_segsort2d(const int* __restrict__ evOff,
         rank += q < p[r] ? 1 : 0;
       }
       const int out = start + bs + rank;
-      keyOut[out] = (uint32_t)(L * k2dBinsPerLayer + b);
+      // keys only where k_reach2d reads them: the first and last position of every 256-point tile
+      // (blocks start every kSortBlock points from the event's start, make_blocks) and the event's
+      // last position
+      const int rel = (out - evBegin) & (kThreads - 1);
+      if (rel == 0 || rel == kThreads - 1 || out == evEnd - 1) keyOut[out] = (uint32_t)(L * k2dBinsPerLayer + b);
       xyw[out] = v[r];
     }
   }
@@ -429,8 +434,8 @@ struct TableView {
   bool staged;
 };
 
-__device__ __forceinline__ void make_runs(const TableView& T, uint32_t key, float xi, float yi, float d, Runs& r) {
-  r.L = (int)(key / (uint32_t)k2dBinsPerLayer);
+__device__ __forceinline__ void make_runs(const TableView& T, int L, float xi, float yi, float d, Runs& r) {
+  r.L = L;
   r.xlo = xbin2d(xi - d);
   r.xhi = xbin2d(xi + d);
   r.ylo = ybin2d(yi - d);
@@ -531,13 +536,15 @@ __device__ __forceinline__ void compute_reach(const int4* __restrict__ reach, in
 
 // The table window [tlo, tlo + tn) of a block: its raw 16-bit entries from the 16-byte word that
 // holds tlo (the rows of an event's table and each layer's 10^4 entries start on 16-byte words), as
-// one 16-byte load per thread, and the starts of its layers (up to 101, one load per thread). It is
-// staged when it spans at most kTab16Stage entries (3 KB of LDS).
+// one 16-byte load per thread, staged when it spans at most kTab16Stage entries (3 KB of LDS), and
+// the starts of its layers (up to 101, one load per thread), staged always. The window's first entry
+// is in the layer of the block's first point (L0) and its last entry is in the layer after the last
+// point's (L1, from thi = L1 * 10^4 + (last column + 1) * 100, reach_columns).
 constexpr int kTab16Stage = 1536;
 static_assert(kTab16Stage % 8 == 0 && kTab16Stage / 8 <= kThreads && k2dLayers + 1 <= kThreads,
               "one load per thread of the smallest block");
 struct TabWindow {
-  int base, n, L0, nStarts;
+  int base, n, L0, L1, nStarts;
   bool fits;
 };
 __device__ __forceinline__ TabWindow table_window(int tlo, int tn) {
@@ -546,14 +553,24 @@ __device__ __forceinline__ TabWindow table_window(int tlo, int tn) {
   w.n = tlo + tn - w.base;
   w.fits = w.n <= kTab16Stage;
   w.L0 = tlo / k2dBinsPerLayer;
-  const int Lend = (tlo + tn - 1) / k2dBinsPerLayer + 1;  // one past the last entry's layer
+  w.L1 = (tlo + tn - 2) / k2dBinsPerLayer;
+  const int Lend = (tlo + tn - 1) / k2dBinsPerLayer + 1;  // one past the last entry's layer (> L1)
   w.nStarts = (Lend < k2dLayers ? Lend : k2dLayers) - w.L0 + 1;
   return w;
 }
 __device__ __forceinline__ void stage_table(const BinTable2D& T, const TabWindow& w, uint16_t* st, int* layerStart) {
   const int tid = (int)threadIdx.x;
-  if (tid < (w.n + 7) / 8) reinterpret_cast<uint4*>(st)[tid] = reinterpret_cast<const uint4*>(T.t16 + w.base)[tid];
+  if (w.fits && tid < (w.n + 7) / 8)
+    reinterpret_cast<uint4*>(st)[tid] = reinterpret_cast<const uint4*>(T.t16 + w.base)[tid];
   if (tid < w.nStarts) layerStart[tid] = T.segB[w.L0 + tid];
+}
+// The layer of the block's sorted position p (layers L0..L1, starts staged by stage_table): a point
+// of layer L has start(L) <= p < start(L + 1) <= start(L'') for every L'' > L, so the number of
+// starts of layers L0 + 1..L1 that are <= p is L - L0. Most blocks lie in one layer (no step).
+__device__ __forceinline__ int layer_at(const TabWindow& w, const int* layerStart, int p) {
+  int L = w.L0;
+  for (int j = 1; j <= w.L1 - w.L0; ++j) L += p >= layerStart[j] ? 1 : 0;
+  return L;
 }
 
 template <bool STAGED>
@@ -811,7 +828,6 @@ __device__ __forceinline__ float density_run(int p, float xi, float yi, int s, i
 
 template <int NT, int ITEMS, int STAGE>
 __global__ __launch_bounds__(NT) CLUE_WAVES_PER_EU(kWavesPerEu2D<NT, ITEMS, STAGE>) void k_density2d(const BlockDesc* __restrict__ blocks,
-                                                        const uint32_t* __restrict__ keyS,
                                                         const float4* __restrict__ xyw, BinTables2D tabs,
                                                         const int4* __restrict__ reach, float dc, float dc2,
                                                         uint32_t* __restrict__ rhoKey, float* __restrict__ rhoOut) {
@@ -833,19 +849,20 @@ __global__ __launch_bounds__(NT) CLUE_WAVES_PER_EU(kWavesPerEu2D<NT, ITEMS, STAG
   if (staged) stage_range_half<NT, STAGE>(xyw, s_st, rlo, rhi);
   const TabWindow win = table_window(tlo, tn);
   const TableView tab{T, s_tab, s_layerStart, win.base, win.L0, win.fits};
-  if (tab.staged) stage_table(T, win, s_tab, s_layerStart);
+  stage_table(T, win, s_tab, s_layerStart);
   float4 me[ITEMS];
-  uint32_t key[ITEMS];
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
     const int p = b0 + r * NT + (int)threadIdx.x;
     me[r] = xyw[p < b1 ? p : b0];
-    key[r] = keyS[p < b1 ? p : b0];
   }
   __syncthreads();
   Runs runs[ITEMS];
 #pragma unroll
-  for (int r = 0; r < ITEMS; ++r) make_runs(tab, key[r], me[r].x, me[r].y, dc, runs[r]);
+  for (int r = 0; r < ITEMS; ++r) {
+    const int p = b0 + r * NT + (int)threadIdx.x;
+    make_runs(tab, layer_at(win, s_layerStart, p < b1 ? p : b0), me[r].x, me[r].y, dc, runs[r]);
+  }
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
     const int p = b0 + r * NT + (int)threadIdx.x;
@@ -1020,17 +1037,17 @@ __device__ __forceinline__ int nearest_higher_of(uint64_t qkey, float xi, float 
 }
 
 // delta / nearestHigher plus the seed/outlier decision of kernel_findAndAssign_clusters
-// (CLUEAlgoKernels.h:108-137).
+// (CLUEAlgoKernels.h:108-137); each seed also sets its bit in the batch's seed words (cleared before
+// the launch), from which the labels step ranks the seeds.
 template <int NT, int ITEMS, int STAGE>
 __global__ __launch_bounds__(NT) CLUE_WAVES_PER_EU(kWavesPerEu2D<NT, ITEMS, STAGE>) void k_nh2d(const BlockDesc* __restrict__ blocks,
-                                                   const uint32_t* __restrict__ keyS,
                                                    const float4* __restrict__ xyw, const uint32_t* __restrict__ rhoKey,
                                                    BinTables2D tabs,
                                                    const int4* __restrict__ reach, const int* __restrict__ evOff,
                                                    float dc, float rhoc,
                                                    float odf, float dm2, float dm2up, float* __restrict__ deltaOut,
                                                    int32_t* __restrict__ nhOut, int32_t* __restrict__ isSeedOut,
-                                                   int* __restrict__ parent) {
+                                                   int* __restrict__ parent, uint64_t* __restrict__ seedWord) {
   using Shape = CompShape<NT, ITEMS>;
   static_assert(STAGE <= kStageMax, "staged byte offsets take 16 bits");
   __shared__ float4 s_st[STAGE + 1];  // + the spare element a masked second candidate may read
@@ -1052,19 +1069,20 @@ __global__ __launch_bounds__(NT) CLUE_WAVES_PER_EU(kWavesPerEu2D<NT, ITEMS, STAG
   if (staged) stage_range_nh<NT, STAGE>(src, s_st, rlo, rhi);
   const TabWindow win = table_window(tlo, tn);
   const TableView tab{T, s_tab, s_layerStart, win.base, win.L0, win.fits};
-  if (tab.staged) stage_table(T, win, s_tab, s_layerStart);
+  stage_table(T, win, s_tab, s_layerStart);
   float4 me[ITEMS];
-  uint32_t key[ITEMS];
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
     const int p = b0 + r * NT + (int)threadIdx.x;
     me[r] = src.at(p < b1 ? p : b0);
-    key[r] = keyS[p < b1 ? p : b0];
   }
   __syncthreads();
   Runs runs[ITEMS];
 #pragma unroll
-  for (int r = 0; r < ITEMS; ++r) make_runs(tab, key[r], me[r].x, me[r].y, dm, runs[r]);
+  for (int r = 0; r < ITEMS; ++r) {
+    const int p = b0 + r * NT + (int)threadIdx.x;
+    make_runs(tab, layer_at(win, s_layerStart, p < b1 ? p : b0), me[r].x, me[r].y, dm, runs[r]);
+  }
 #pragma unroll
   for (int r = 0; r < ITEMS; ++r) {
     const int p = b0 + r * NT + (int)threadIdx.x;
@@ -1083,6 +1101,7 @@ __global__ __launch_bounds__(NT) CLUE_WAVES_PER_EU(kWavesPerEu2D<NT, ITEMS, STAG
       const bool seed = (delta > dc) && (rhoi >= rhoc);
       const bool outlier = (delta > dm) && (rhoi < rhoc);
       isSeedOut[i] = seed ? 1 : 0;
+      if (seed) atomicOr(reinterpret_cast<unsigned long long*>(seedWord + (i >> 6)), 1ull << (i & 63));
       // Followers point at their nearest higher; seeds, outliers (and a point without a nearest
       // higher, which the reference cannot register) are roots.
       parent[i] = (seed || outlier || nhg < 0) ? i : nhg;

@@ -273,10 +273,13 @@ inline SeedRank seed_rank_carve(Carver& cv, int64_t N) {
 }
 
 // clusterIndex[i] = isSeed[root(i)] ? (#seeds of the event before root) : -1, where root(i) is
-// reached by following parent[] (parent[i] == i at roots). parent[] is shortened in place while
-// walking. `part` needs scan_partials(seed_tiles(N)) ints.
-void launch_labels(const BlockDesc* blocksDev, int nBlocks, int* parent, const int* isSeed, int64_t N,
+// reached by following parent[] (parent[i] == i at roots). `part` needs scan_partials(seed_tiles(N))
+// ints. isSeed == nullptr: the seed words are already set (cleared by clear_seed_words, then one bit
+// per seed set by the caller's kernels).
+void launch_labels(const BlockDesc* blocksDev, int nBlocks, const int* parent, const int* isSeed, int64_t N,
                    const int* evOffDev, int32_t* label, const SeedRank& sr, int* part, hipStream_t s);
+// Zero the seed words of N points on the stream (a fill kernel, no memset node in a captured graph).
+void clear_seed_words(const SeedRank& sr, int64_t N, hipStream_t s);
 
 // Fill an int buffer on the stream with a kernel (graph-capture friendly, no memset nodes).
 void fill_i32(int* p, int value, int64_t n, hipStream_t s);

@@ -46,13 +46,21 @@ void build_table(const uint32_t* keySorted, const BlockDesc* blocksDev, int nBlo
                                                stride, table);
 }
 
-void launch_labels(const BlockDesc* blocksDev, int nBlocks, int* parent, const int* isSeed, int64_t N,
+void launch_labels(const BlockDesc* blocksDev, int nBlocks, const int* parent, const int* isSeed, int64_t N,
                    const int* evOffDev, int32_t* label, const SeedRank& sr, int* part, hipStream_t s) {
   if (nBlocks <= 0 || N <= 0) return;
   const int tiles = seed_tiles(N);
-  k_seed_bits<<<tiles, kThreads, 0, s>>>(isSeed, N, sr);
+  if (isSeed)
+    k_seed_bits<<<tiles, kThreads, 0, s>>>(isSeed, N, sr);
+  else
+    k_seed_prefix<<<(tiles * 32 + kThreads - 1) / kThreads, kThreads, 0, s>>>((int64_t)tiles * 32, sr);
   scan_exclusive(sr.tilePre, sr.tilePre, tiles, part, s);
   k_labels<<<nBlocks, kThreads, 0, s>>>(blocksDev, parent, sr, evOffDev, label);
+}
+
+void clear_seed_words(const SeedRank& sr, int64_t N, hipStream_t s) {
+  if (N <= 0) return;
+  fill_i32(reinterpret_cast<int*>(sr.word), 0, (int64_t)seed_tiles(N) * 32 * 2, s);
 }
 
 void fill_i32(int* p, int value, int64_t n, hipStream_t s) {

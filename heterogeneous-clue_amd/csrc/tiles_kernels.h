@@ -209,6 +209,23 @@ __global__ __launch_bounds__(kThreads) void k_seed_bits(const int* __restrict__ 
   }
 }
 
+// The same prefixes from seed words a kernel has already set (bit i % 64 of word i / 64 for each
+// seed i, over words cleared beforehand): one thread per word, two tiles of 32 words per wave.
+__global__ __launch_bounds__(kThreads) void k_seed_prefix(int64_t nWords, SeedRank sr) {
+  static_assert(kSeedTile / kWave == 32, "two seed tiles per wave");
+  const int lane = lane_id();
+  const int64_t w = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  const int v = w < nWords ? __popcll(sr.word[w]) : 0;
+  int inc = wave_inclusive_sum(v);
+  // restart at the wave's second tile; the shuffle runs on every lane (a lane it reads must be active)
+  const int firstTile = __shfl(inc, 31, kWave);
+  inc -= lane >= 32 ? firstTile : 0;
+  if (w < nWords) {
+    sr.wordPre[w] = (uint16_t)(inc - v);
+    if ((lane & 31) == 31) sr.tilePre[w / 32] = inc;
+  }
+}
+
 __device__ __forceinline__ bool seed_of(const SeedRank& sr, int i) {
   return (sr.word[i >> 6] >> (i & 63)) & 1ull;
 }
@@ -216,15 +233,17 @@ __device__ __forceinline__ int rank_of(const SeedRank& sr, int i) {
   return sr.tilePre[i / kSeedTile] + sr.wordPre[i >> 6] + __popcll(sr.word[i >> 6] & ((1ull << (i & 63)) - 1ull));
 }
 
-__global__ __launch_bounds__(kThreads) void k_labels(const BlockDesc* __restrict__ blocks, int* parent,
+__global__ __launch_bounds__(kThreads) void k_labels(const BlockDesc* __restrict__ blocks,
+                                                     const int* __restrict__ parent,
                                                      SeedRank sr, const int* __restrict__ evOff,
                                                      int32_t* __restrict__ label) {
   const BlockDesc bd = blocks[xcd_block((int)blockIdx.x, (int)gridDim.x)];
   const int rank0 = rank_of(sr, evOff[bd.event]);
   // The thread's kSortItems points walk to their roots together: each round issues the next parent
   // read of every unfinished walk before using any, so the dependent loads of different walks
-  // overlap. Every value read from parent[] (old or already shortened by another thread) is an
-  // ancestor of the point read, and roots never change, so each walk ends.
+  // overlap. parent[] is only read: every follower's parent is strictly higher in the (rho, index)
+  // order and roots point at themselves, so each walk ends. (Writing the roots back into parent[]
+  // to shorten later walks costs more in writes than it saves.)
   int cur[kSortItems];
   bool walking[kSortItems];
 #pragma unroll
@@ -249,10 +268,7 @@ __global__ __launch_bounds__(kThreads) void k_labels(const BlockDesc* __restrict
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     const int i = bd.begin + r * kThreads + (int)threadIdx.x;
-    if (i < bd.end) {
-      parent[i] = cur[r];
-      label[i] = seed_of(sr, cur[r]) ? rank_of(sr, cur[r]) - rank0 : -1;
-    }
+    if (i < bd.end) label[i] = seed_of(sr, cur[r]) ? rank_of(sr, cur[r]) - rank0 : -1;
   }
 }
 

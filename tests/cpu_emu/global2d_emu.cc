@@ -88,19 +88,20 @@ int main(int argc, char** argv) {
     const int dv = argc > 3 ? atoi(argv[3]) : 0, nv = argc > 4 ? atoi(argv[4]) : 0;
     dispatch(dv, [&](auto T, auto I, auto S) {
       emu::launch(CompShape<T, I>::kPerSort * nb, T, [&] {
-        k_density2d<T, I, S>(blocks.data(), keyS.data(), xyw.data(), tabs, reachD.data(), dc, dc2,
+        k_density2d<T, I, S>(blocks.data(), xyw.data(), tabs, reachD.data(), dc, dc2,
                              rhoKey.data(), rho.data());
       });
     });
     int* parent = reinterpret_cast<int*>(lg.data());
+    SeedRankHost seeds(N);
+    std::fill(seeds.word.begin(), seeds.word.end(), 0ull);  // clear_seed_words
     dispatch(nv, [&](auto T, auto I, auto S) {
       emu::launch(CompShape<T, I>::kPerSort * nb, T, [&] {
-        k_nh2d<T, I, S>(blocks.data(), keyS.data(), xyw.data(), rhoKey.data(), tabs, reachN.data(),
-                        evOff.data(), dc, rhoc, odf, dm2, dm2up, delta.data(), nh.data(), isSeed.data(), parent);
+        k_nh2d<T, I, S>(blocks.data(), xyw.data(), rhoKey.data(), tabs, reachN.data(), evOff.data(), dc, rhoc, odf,
+                        dm2, dm2up, delta.data(), nh.data(), isSeed.data(), parent, seeds.word.data());
       });
     });
-    SeedRankHost seeds(N);
-    launch_labels(blocks.data(), nb, parent, isSeed.data(), N, evOff.data(), ci.data(), seeds.view(), part.data());
+    launch_labels(blocks.data(), nb, parent, nullptr, N, evOff.data(), ci.data(), seeds.view(), part.data());
   }
   write_output(argv[2], err, 0, rho, delta, nh, isSeed, ci);
   return 0;

@@ -37,11 +37,14 @@ inline void build_table(const uint32_t* key, const BlockDesc* blocks, int nBlock
               [&] { k_table(key, evOff, tileStart, tiles, nKeys, binsPerLayer, layerReach, stride, table); });
 }
 
-inline void launch_labels(const BlockDesc* blocks, int nBlocks, int* parent, const int* isSeed, int64_t N,
+inline void launch_labels(const BlockDesc* blocks, int nBlocks, const int* parent, const int* isSeed, int64_t N,
                           const int* evOff, int32_t* label, const SeedRank& sr, int* part) {
   if (nBlocks <= 0 || N <= 0) return;
   const int tiles = seed_tiles(N);
-  emu::launch(tiles, kThreads, [&] { k_seed_bits(isSeed, N, sr); });
+  if (isSeed)
+    emu::launch(tiles, kThreads, [&] { k_seed_bits(isSeed, N, sr); });
+  else
+    emu::launch((tiles * 32 + kThreads - 1) / kThreads, kThreads, [&] { k_seed_prefix((int64_t)tiles * 32, sr); });
   scan_exclusive(sr.tilePre, sr.tilePre, tiles, part);
   emu::launch(nBlocks, kThreads, [&] { k_labels(blocks, parent, sr, evOff, label); });
 }
