@@ -90,6 +90,12 @@ constexpr int kSegThreadsL = 1024;
 constexpr int kSegThreadsH = 512;
 constexpr int kSegNone = 0;  // "no upper bound": the class of the segments above kSegCap
 constexpr int kSegIlp = 4;   // points of a thread taken together in each loop over a segment
+// Points of a thread (positions tid + r * NT, r < kSegHold) whose records the kSegCap class keeps in
+// registers from the count pass to the output pass, so that they are read once; with the LDS of two
+// workgroups per CU the registers of 8 waves per SIMD (64 each) hold five records.
+constexpr int kSegHold = 5;
+template <int CAP>
+constexpr int kSegHoldOf = CAP == kSegCap ? kSegHold : 0;
 
 __device__ __forceinline__ int bin2d(const float4& v) { return xbin2d(v.x) * k2dRows + ybin2d(v.y); }
 
@@ -203,9 +209,11 @@ struct SegCounts {
 //   step's own work on that bin. The LDS classes keep each point's bin and each slot's point in LDS
 //   (u16); the class above kSegCap keeps the slots in `scr` (the segment's own range of a global
 //   array) and recomputes bins. Empty segments take the smallest class: their table entries are
-//   written too.
+//   written too. The kSegCap class ranks point by point instead, its first kSegHold * NT points from
+//   the records it kept in registers since the count pass (the rest read again, in index order):
+//   its segments (up to 160 KB) are evicted from L2 between the two passes.
 template <int NT, int LO, int CAP>
-__global__ __launch_bounds__(NT) void k_segsort2d(const int* __restrict__ evOff, const int* __restrict__ segB,
+__global__ __launch_bounds__(NT) CLUE_WAVES_PER_EU(kSegHoldOf<CAP> > 0 ? 8 : 1) void k_segsort2d(const int* __restrict__ evOff, const int* __restrict__ segB,
                                                   const float4* __restrict__ lg, int* __restrict__ scr,
                                                   uint32_t* __restrict__ keyOut, float4* __restrict__ xyw,
                                                   uint16_t* __restrict__ t16, int* __restrict__ t32, int64_t stride) {
@@ -230,8 +238,26 @@ __global__ __launch_bounds__(NT) void k_segsort2d(const int* __restrict__ evOff,
   }
   __syncthreads();
   // bin counts; every loop over the points takes kSegIlp of a thread's points at once, all their
-  // loads issued before any is used
-  for (int p0 = tid; p0 < n; p0 += kSegIlp * NT) {
+  // loads issued before any is used (the held points first, all at once)
+  constexpr int kHold = kSegHoldOf<CAP>;
+  float4 held[kHold > 0 ? kHold : 1];
+  if constexpr (kHold > 0) {
+#pragma unroll
+    for (int r = 0; r < kHold; ++r) {
+      const int p = tid + r * NT;
+      held[r] = lg[start + (p < n ? p : 0)];
+    }
+#pragma unroll
+    for (int r = 0; r < kHold; ++r) {
+      const int p = tid + r * NT;
+      if (p < n) {
+        const int b = bin2d(held[r]);
+        s_bin[p] = (uint16_t)b;
+        cnt.take(b);
+      }
+    }
+  }
+  for (int p0 = tid + kHold * NT; p0 < n; p0 += kSegIlp * NT) {
     float4 v[kSegIlp];
 #pragma unroll
     for (int r = 0; r < kSegIlp; ++r) {
@@ -338,7 +364,46 @@ __global__ __launch_bounds__(NT) void k_segsort2d(const int* __restrict__ evOff,
     }
   }
   __syncthreads();
-  // the count words now hold each bin's end; rank each slot's point inside its bin by position
+  // the count words now hold each bin's end; rank each point inside its bin by position
+  const auto emit = [&](int b, int bs, int rank, const float4& v) {
+    const int out = start + bs + rank;
+    // keys only where k_reach2d reads them: the first and last position of every 256-point tile
+    // (blocks start every kSortBlock points from the event's start, make_blocks) and the event's
+    // last position
+    const int rel = (out - evBegin) & (kThreads - 1);
+    if (rel == 0 || rel == kThreads - 1 || out == evEnd - 1) keyOut[out] = (uint32_t)(L * k2dBinsPerLayer + b);
+    xyw[out] = v;
+  };
+  if constexpr (kHold > 0) {
+    // by point: the rank of p is the number of its bin's slots holding a smaller position
+    const auto emit_point = [&](int p, const float4& v) {
+      const int b = (int)s_bin[p];
+      const int bs = b > 0 ? cnt.get(b - 1) : 0, be = cnt.get(b);
+      int rank = 0;
+      for (int f = bs; f < be; ++f) rank += (int)s_slot[f] < p ? 1 : 0;
+      emit(b, bs, rank, v);
+    };
+#pragma unroll
+    for (int r = 0; r < kHold; ++r) {
+      const int p = tid + r * NT;
+      if (p < n) emit_point(p, held[r]);
+    }
+    for (int p0 = tid + kHold * NT; p0 < n; p0 += kSegIlp * NT) {
+      float4 v[kSegIlp];
+#pragma unroll
+      for (int r = 0; r < kSegIlp; ++r) {
+        const int p = p0 + r * NT;
+        v[r] = lg[start + (p < n ? p : 0)];
+      }
+#pragma unroll
+      for (int r = 0; r < kSegIlp; ++r) {
+        const int p = p0 + r * NT;
+        if (p < n) emit_point(p, v[r]);
+      }
+    }
+    return;
+  }
+  // by slot: each slot's point counts the points of its bin with a smaller position
   for (int j0 = tid; j0 < n; j0 += kSegIlp * NT) {
     int p[kSegIlp];
     float4 v[kSegIlp];
@@ -370,13 +435,7 @@ __global__ __launch_bounds__(NT) void k_segsort2d(const int* __restrict__ evOff,
           q = scr[start + f];
         rank += q < p[r] ? 1 : 0;
       }
-      const int out = start + bs + rank;
-      // keys only where k_reach2d reads them: the first and last position of every 256-point tile
-      // (blocks start every kSortBlock points from the event's start, make_blocks) and the event's
-      // last position
-      const int rel = (out - evBegin) & (kThreads - 1);
-      if (rel == 0 || rel == kThreads - 1 || out == evEnd - 1) keyOut[out] = (uint32_t)(L * k2dBinsPerLayer + b);
-      xyw[out] = v[r];
+      emit(b, bs, rank, v[r]);
     }
   }
 }

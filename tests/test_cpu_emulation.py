@@ -127,11 +127,11 @@ def test_emulated_global_batch(emulator, tmp_path, oracle_lib):
 
 def large_layer_event(seed=43):
     """One event with a layer in each size class of k_segsort2d: 12000 points on layer 3 (above the
-    10240 it keeps in LDS), 5000 on layer 50 (the middle class), both with one densely filled bin, and
-    300 on layers 0 and 99."""
+    10240 it keeps in LDS), 8000 on layer 50 (the middle class, more than the 5120 whose records it
+    keeps in registers), both with one densely filled bin, and 300 on layers 0 and 99."""
     rng = np.random.Generator(np.random.PCG64(seed))
     xs, ys, ls = [], [], []
-    for lay, n, dense in ((3, 12000, 2000), (50, 5000, 700), (0, 300, 0), (99, 300, 0)):
+    for lay, n, dense in ((3, 12000, 2000), (50, 8000, 700), (0, 300, 0), (99, 300, 0)):
         xs += [rng.uniform(-250, 250, n - dense), rng.uniform(10.0, 14.9, dense)]
         ys += [rng.uniform(-250, 250, n - dense), rng.uniform(-4.9, -0.1, dense)]
         ls.append(np.full(n, lay))
