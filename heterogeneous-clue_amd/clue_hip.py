@@ -17,7 +17,7 @@ CLUE_OK = 0
 ERRORS = {-1: "CLUE_ERR_ARG", -2: "CLUE_ERR_HIP", -3: "CLUE_ERR_BUDGET", -4: "CLUE_ERR_LAYER", -5: "CLUE_ERR_BIN",
           -6: "CLUE_ERR_TIMEOUT", -7: "CLUE_ERR_CAPACITY"}
 PATH_AUTO, PATH_FUSED, PATH_GLOBAL = 0, 1, 2
-PLAN_GRAPH, PLAN_AUTOTUNE = 1, 2
+PLAN_GRAPH, PLAN_AUTOTUNE, PLAN_INDEX_PASS = 1, 2, 4
 FUSED_CAPACITY = 10112
 NOALLOC_MAX_POINTS = 1048576
 
@@ -34,7 +34,7 @@ EXPORTS = [
     "clue_plan_workspace_bytes", "clue_selftest_arith", "clue_server_create", "clue_server_destroy",
     "clue_server_submit2d", "clue_server_submit3d", "clue_server_wait", "clue_server_stats",
     "clue_server_last_error", "clue_set_path", "clue_plan_path", "clue_plan_tuning", "clue2d_run_device_noalloc",
-    "clue_plan_kernel_ms",
+    "clue_plan_kernel_ms", "clue_plan_index_pass",
 ]
 
 
@@ -119,6 +119,7 @@ def load_library(path: Path = LIB_PATH) -> C.CDLL:
         "clue_server_last_error": (C.c_char_p, [VP]),
         "clue_set_path": (I32, [VP, I32, I32]),
         "clue_plan_path": (I32, [VP]),
+        "clue_plan_index_pass": (I32, [VP]),
         "clue_plan_tuning": (I32, [VP, VP]),
         "clue_plan_kernel_ms": (C.c_float, [VP, I32, I32, VP]),
     }
@@ -265,10 +266,11 @@ class Device:
                                                    *[dout[k].ptr for k in FIELDS_2D_OUT], stream))
 
     def clue2d_plan(self, din: dict, dout: dict, offs: np.ndarray, p: Clue2DParams | None = None,
-                    use_graph: bool = True, autotune: bool = False) -> "Plan":
+                    use_graph: bool = True, autotune: bool = False, index_pass: bool = False) -> "Plan":
         p = p or params2d()
         offs = np.ascontiguousarray(offs, dtype=np.int64)
-        flags = (PLAN_GRAPH if use_graph else 0) | (PLAN_AUTOTUNE if autotune else 0)
+        flags = ((PLAN_GRAPH if use_graph else 0) | (PLAN_AUTOTUNE if autotune else 0)
+                 | (PLAN_INDEX_PASS if index_pass else 0))
         h = self.lib.clue2d_plan_create(self.ctx, C.byref(p), offs.size - 1, offs.ctypes.data,
                                         *[din[k].ptr for k in FIELDS_2D_IN], *[dout[k].ptr for k in FIELDS_2D_OUT],
                                         flags)
@@ -418,7 +420,12 @@ class Plan:
         out = np.zeros(4, np.int32)
         self.dev._chk(self.dev.lib.clue_plan_tuning(self.h, out.ctypes.data))
         return {"density_points": int(out[0]), "density_stage": int(out[1]), "nh_points": int(out[2]),
-                "nh_stage": int(out[3])}
+                "nh_stage": int(out[3]), "nh_index_pass": self.index_pass}
+
+    @property
+    def index_pass(self) -> bool:
+        """CLUE2D: whether the nearest-higher step uses the index-pass output form."""
+        return bool(self.dev.lib.clue_plan_index_pass(self.h))
 
     @property
     def path(self) -> int:

@@ -35,6 +35,7 @@ struct clue_plan {
   Clue3DParams p3{};
   bool useGraph = false;
   bool tune = false;   // CLUE_PLAN_AUTOTUNE requested and not done yet
+  bool indexPass = false;  // CLUE_PLAN_INDEX_PASS: the nearest-higher step's index-pass output form
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -270,6 +271,7 @@ int plan_setup_global(clue_plan* pl) {
   if (rc) return rc;
   clue2d_carve(pl->ws.base, N, E, nb, pl->b2);
   pl->b2.err = c->dErr;
+  pl->b2.nhIndexPass = pl->indexPass ? 1 : 0;
   std::vector<BlockDesc> blocks;
   return upload_tables(c, pl->offsets.data(), E, pl->b2.blocks, pl->b2.evOff, blocks);
 }
@@ -335,15 +337,32 @@ int plan_autotune(clue_plan* pl, hipStream_t s) {
     for (int v = 1; v < nVariants; ++v)
       if (ms[v] < ms[best[which]]) best[which] = v;
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   if (d3) {
     pl->b3.densityVariant = best[0];
     pl->b3.nhVariant = best[1];
   } else {
     pl->b2.densityVariant = best[0];
     pl->b2.nhVariant = best[1];
+    // the output form of the nearest-higher step: both forms of that kernel plus its labels step,
+    // timed the same way (hits that arrive grouped by position favour the direct writes, hits in an
+    // order unrelated to position the index pass)
+    float ms[2] = {};
+    for (int round = 0; round <= kTuneRounds; ++round) {
+      for (int v = 0; v < 2; ++v) {
+        HIP_TRY(c, hipEventRecord(e0, s));
+        clue2d_enqueue_step(pl->b2, pl->io2, pl->p2, 2, v, s);
+        HIP_TRY(c, hipEventRecord(e1, s));
+        HIP_TRY(c, hipEventSynchronize(e1));
+        float t = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&t, e0, e1));
+        if (round == 1 || (round > 1 && t < ms[v])) ms[v] = t;
+      }
+    }
+    pl->b2.nhIndexPass = ms[1] < ms[0] ? 1 : 0;
+    pl->indexPass = pl->b2.nhIndexPass != 0;
   }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   pl->tune = false;
   return CLUE_OK;
 }
@@ -747,6 +766,7 @@ clue_plan* clue2d_plan_create(clue_ctx* c, const clue2d_params* p, int32_t E, co
   pl->p2 = to2d(p);
   pl->useGraph = (use_graph & CLUE_PLAN_GRAPH) != 0;
   pl->tune = (use_graph & CLUE_PLAN_AUTOTUNE) != 0;
+  pl->indexPass = (use_graph & CLUE_PLAN_INDEX_PASS) != 0;
   if (c->path2d == CLUE_PATH_GLOBAL) {
     pl->route2d = CLUE_PATH_GLOBAL;
     if (plan_setup_global(pl) != CLUE_OK) {
@@ -931,6 +951,11 @@ void clue_plan_destroy(clue_plan* pl) {
 int32_t clue_plan_path(const clue_plan* pl) {
   if (!pl) return CLUE_ERR_ARG;
   return pl->dim == 2 ? pl->route2d : CLUE_PATH_GLOBAL;
+}
+
+int32_t clue_plan_index_pass(const clue_plan* pl) {
+  if (!pl) return CLUE_ERR_ARG;
+  return pl->dim == 2 && pl->indexPass ? 1 : 0;
 }
 
 int32_t clue_plan_tuning(const clue_plan* pl, int32_t* out4) {

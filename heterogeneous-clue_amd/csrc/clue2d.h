@@ -32,11 +32,14 @@ struct Clue2DBuffers {
   const BlockDesc* blocks = nullptr;  // device
   const int* evOff = nullptr;         // device, E + 1 (int32 offsets)
   uint32_t* keyS = nullptr;  // tile order: bin keys (written at each 256-point tile's first and last position)
-  int* scr = nullptr;        // slots of layer segments too large for LDS (k_segsort2d)
-  float4* lg = nullptr;      // layer order: {x, y, weight, point index as int bits}; later the parent array
+  int* scr = nullptr;        // slots of layer segments too large for LDS (k_segsort2d); later the
+                             // tile-order parent links of the index-pass output form
+  float4* lg = nullptr;      // layer order: {x, y, weight, point index as int bits}; later the parent
+                             // array, or the index-order records of the index-pass output form
   int* segB = nullptr;       // per event, the first position of each layer (E * (k2dLayers + 1))
   float4* xyw = nullptr;   // tile order: {x, y, weight, point index as int bits}
   uint32_t* rhoKey = nullptr;  // tile order: rho as an order-preserving unsigned key (rho_key)
+  int2* lab = nullptr;         // tile order, index-pass output form: {label, rho bits}
   int4* reachD = nullptr;  // per 256-point tile: staging / table reach of the density boxes
   int4* reachN = nullptr;  // the same for the nearest-higher boxes
   int* hist = nullptr;
@@ -49,6 +52,11 @@ struct Clue2DBuffers {
   int errStride = 0;
   int densityVariant = 0;  // (points per thread, staging) variant of the density kernel (kVariants2D)
   int nhVariant = 0;       // the same for the nearest-higher kernel
+  // Output form of the neighbour steps: 0 writes rho (density), delta, nearestHigher, isSeed and the
+  // links (nearest higher) at each point's index; 1 keeps rho and the links in tile order and writes
+  // one index-order record per point (k_nh2d's INDEX_PASS), and the labels step writes all five
+  // outputs in index order.
+  int nhIndexPass = 0;
 };
 
 // Caller-owned device inputs/outputs, packed over the batch (nearestHigher and clusterIndex are
@@ -68,10 +76,11 @@ struct Clue2DIO {
 constexpr int64_t clue2d_table_stride() { return ((int64_t)k2dKeys + 1 + 63) / 64 * 64; }
 
 void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DParams& p, hipStream_t s);
-// The density (which = 0) or nearest-higher (which = 1) kernel alone with the given variant, on a
-// batch clue2d_enqueue has run on: both read only what earlier steps left and rewrite their own
-// outputs with the same values (the seed bits are OR-ed in again), so the autotuner can time them
-// in place.
+// The density (which = 0) or nearest-higher (which = 1) kernel alone with the given variant, or the
+// density, nearest-higher and labels steps (variants b.densityVariant, b.nhVariant) in output form
+// `variant` (which = 2, see nhIndexPass), on a batch clue2d_enqueue has run on: each reads only
+// what earlier steps left and rewrites its outputs with the same values (the seed bits are OR-ed in
+// again), so the autotuner can time them in place.
 void clue2d_enqueue_step(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DParams& p, int which, int variant,
                          hipStream_t s);
 constexpr int kClue2DVariants = 6;

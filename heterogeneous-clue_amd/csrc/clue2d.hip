@@ -54,41 +54,83 @@ using namespace g2d;
 namespace {
 
 template <int V, class... A>
-void density_v(int nBlocks, hipStream_t s, A... a) {
+void density_v(bool indexPass, int nBlocks, hipStream_t s, A... a) {
   constexpr Variant2D v = kVariants2D[V];
-  k_density2d<v.threads, v.items, v.stage><<<CompShape<v.threads, v.items>::kPerSort * nBlocks, v.threads, 0, s>>>(a...);
+  const int grid = CompShape<v.threads, v.items>::kPerSort * nBlocks;
+  if (indexPass)
+    k_density2d<v.threads, v.items, v.stage, false><<<grid, v.threads, 0, s>>>(a...);
+  else
+    k_density2d<v.threads, v.items, v.stage, true><<<grid, v.threads, 0, s>>>(a...);
 }
 template <int V, class... A>
-void nh_v(int nBlocks, hipStream_t s, A... a) {
+void nh_v(bool indexPass, int nBlocks, hipStream_t s, A... a) {
   constexpr Variant2D v = kVariants2D[V];
-  k_nh2d<v.threads, v.items, v.stage><<<CompShape<v.threads, v.items>::kPerSort * nBlocks, v.threads, 0, s>>>(a...);
+  const int grid = CompShape<v.threads, v.items>::kPerSort * nBlocks;
+  if (indexPass)
+    k_nh2d<v.threads, v.items, v.stage, true><<<grid, v.threads, 0, s>>>(a...);
+  else
+    k_nh2d<v.threads, v.items, v.stage, false><<<grid, v.threads, 0, s>>>(a...);
 }
 
 }  // namespace
 
 template <class... A>
-void launch_density2d(int variant, int nBlocks, hipStream_t s, A... a) {
+void launch_density2d(int variant, bool indexPass, int nBlocks, hipStream_t s, A... a) {
   switch (variant) {
-    case 1: density_v<1>(nBlocks, s, a...); break;
-    case 2: density_v<2>(nBlocks, s, a...); break;
-    case 3: density_v<3>(nBlocks, s, a...); break;
-    case 4: density_v<4>(nBlocks, s, a...); break;
-    case 5: density_v<5>(nBlocks, s, a...); break;
-    default: density_v<0>(nBlocks, s, a...); break;
+    case 1: density_v<1>(indexPass, nBlocks, s, a...); break;
+    case 2: density_v<2>(indexPass, nBlocks, s, a...); break;
+    case 3: density_v<3>(indexPass, nBlocks, s, a...); break;
+    case 4: density_v<4>(indexPass, nBlocks, s, a...); break;
+    case 5: density_v<5>(indexPass, nBlocks, s, a...); break;
+    default: density_v<0>(indexPass, nBlocks, s, a...); break;
   }
 }
 template <class... A>
-void launch_nh2d(int variant, int nBlocks, hipStream_t s, A... a) {
+void launch_nh2d(int variant, bool indexPass, int nBlocks, hipStream_t s, A... a) {
   switch (variant) {
-    case 1: nh_v<1>(nBlocks, s, a...); break;
-    case 2: nh_v<2>(nBlocks, s, a...); break;
-    case 3: nh_v<3>(nBlocks, s, a...); break;
-    case 4: nh_v<4>(nBlocks, s, a...); break;
-    case 5: nh_v<5>(nBlocks, s, a...); break;
-    default: nh_v<0>(nBlocks, s, a...); break;
+    case 1: nh_v<1>(indexPass, nBlocks, s, a...); break;
+    case 2: nh_v<2>(indexPass, nBlocks, s, a...); break;
+    case 3: nh_v<3>(indexPass, nBlocks, s, a...); break;
+    case 4: nh_v<4>(indexPass, nBlocks, s, a...); break;
+    case 5: nh_v<5>(indexPass, nBlocks, s, a...); break;
+    default: nh_v<0>(indexPass, nBlocks, s, a...); break;
   }
 }
 static_assert(kNumVariants2D == 6, "launch_density2d / launch_nh2d dispatch six variants");
+
+namespace {
+
+// The neighbour steps and labels in one of the two output forms (Clue2DBuffers::nhIndexPass). The
+// links and the index-order records of the index-pass form live in buffers the tile-order steps no
+// longer need: `lg` (16 bytes per point: the records, or the direct form's links) and `scr` (4
+// bytes per point: the tile-order links).
+void density_step(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DParams& prm, int variant, bool indexPass,
+                  hipStream_t s) {
+  const BinTables2D tabs{b.table16, b.table, b.segB, b.tableStride};
+  launch_density2d(variant, indexPass, b.nBlocks, s, b.blocks, (const float4*)b.xyw, tabs, (const int4*)b.reachD,
+                   prm.dc, prm.dc2, b.rhoKey, io.rho, b.lab);
+}
+void nh_step(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DParams& prm, int variant, bool indexPass,
+             hipStream_t s) {
+  const BinTables2D tabs{b.table16, b.table, b.segB, b.tableStride};
+  int* parent = indexPass ? b.scr : reinterpret_cast<int*>(b.lg);
+  launch_nh2d(variant, indexPass, b.nBlocks, s, b.blocks, (const float4*)b.xyw, (const uint32_t*)b.rhoKey, tabs,
+              (const int4*)b.reachN, b.evOff, prm.dc, prm.rhoc, prm.outlierDeltaFactor, prm.dm2, prm.dm2up, io.delta,
+              io.nearestHigher, io.isSeed, parent, reinterpret_cast<int4*>(b.lg), b.seeds.word);
+}
+void labels_step(const Clue2DBuffers& b, const Clue2DIO& io, bool indexPass, hipStream_t s) {
+  if (!indexPass) {
+    launch_labels(b.blocks, b.nBlocks, reinterpret_cast<const int*>(b.lg), nullptr, b.N, b.evOff, io.clusterIndex,
+                  b.seeds, b.part, s);
+    return;
+  }
+  launch_seed_ranks(b.N, b.seeds, b.part, s);
+  k_labels_tile2d<<<b.nBlocks, kThreads, 0, s>>>(b.blocks, b.scr, b.seeds, b.evOff, b.lab);
+  k_outputs2d<<<b.nBlocks, kThreads, 0, s>>>(b.blocks, reinterpret_cast<const int4*>(b.lg), b.lab, io.rho, io.delta,
+                                             io.nearestHigher, io.isSeed, io.clusterIndex);
+}
+
+}  // namespace
 
 void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DParams& prm, hipStream_t s) {
   if (b.N <= 0 || b.nBlocks <= 0) return;
@@ -109,31 +151,29 @@ void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DPara
                                                              b.table, b.tableStride);
   k_segsort2d<kSegThreadsH, kSegCap, kSegNone><<<nSeg, kSegThreadsH, 0, s>>>(b.evOff, b.segB, b.lg, b.scr, b.keyS, b.xyw, b.table16,
                                                             b.table, b.tableStride);
-  const uint32_t* keyS = b.keyS;
   const BinTables2D tabs{b.table16, b.table, b.segB, b.tableStride};
   const int nTiles = b.nBlocks * kTilesPerSort;
   k_reach2d<<<(nTiles + kReachThreads - 1) / kReachThreads, kReachThreads, 0, s>>>(
-      b.blocks, nTiles, keyS, tabs, prm.dc, prm.outlierDeltaFactor, b.reachD, b.reachN);
-  launch_density2d(b.densityVariant, b.nBlocks, s, b.blocks, (const float4*)b.xyw, tabs, (const int4*)b.reachD, prm.dc,
-                   prm.dc2, b.rhoKey, io.rho);
-  int* parent = reinterpret_cast<int*>(b.lg);
-  launch_nh2d(b.nhVariant, b.nBlocks, s, b.blocks, (const float4*)b.xyw, (const uint32_t*)b.rhoKey, tabs, (const int4*)b.reachN, b.evOff,
-              prm.dc, prm.rhoc, prm.outlierDeltaFactor, prm.dm2, prm.dm2up, io.delta, io.nearestHigher, io.isSeed,
-              parent, b.seeds.word);
-  launch_labels(b.blocks, b.nBlocks, parent, nullptr, b.N, b.evOff, io.clusterIndex, b.seeds, b.part, s);
+      b.blocks, nTiles, b.keyS, tabs, prm.dc, prm.outlierDeltaFactor, b.reachD, b.reachN);
+  const bool ip = b.nhIndexPass != 0;
+  density_step(b, io, prm, b.densityVariant, ip, s);
+  nh_step(b, io, prm, b.nhVariant, ip, s);
+  labels_step(b, io, ip, s);
 }
 
 void clue2d_enqueue_step(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DParams& prm, int which, int variant,
                          hipStream_t s) {
   if (b.N <= 0 || b.nBlocks <= 0) return;
-  const BinTables2D tabs{b.table16, b.table, b.segB, b.tableStride};
-  int* parent = reinterpret_cast<int*>(b.lg);
-  if (which == 0)
-    launch_density2d(variant, b.nBlocks, s, b.blocks, (const float4*)b.xyw, tabs, (const int4*)b.reachD,
-                     prm.dc, prm.dc2, b.rhoKey, io.rho);
-  else
-    launch_nh2d(variant, b.nBlocks, s, b.blocks, (const float4*)b.xyw, (const uint32_t*)b.rhoKey, tabs, (const int4*)b.reachN, b.evOff, prm.dc, prm.rhoc, prm.outlierDeltaFactor, prm.dm2, prm.dm2up, io.delta,
-                io.nearestHigher, io.isSeed, parent, b.seeds.word);
+  const bool ip = b.nhIndexPass != 0;
+  if (which == 0) {
+    density_step(b, io, prm, variant, ip, s);
+  } else if (which == 1) {
+    nh_step(b, io, prm, variant, ip, s);
+  } else {
+    density_step(b, io, prm, b.densityVariant, variant != 0, s);
+    nh_step(b, io, prm, b.nhVariant, variant != 0, s);
+    labels_step(b, io, variant != 0, s);
+  }
 }
 
 void clue2d_variant_shape(int variant, int& points, int& stage) {
@@ -155,6 +195,7 @@ void clue2d_carve(void* base, int64_t N, int E, int nBlocks, Clue2DBuffers& b) {
   b.segB = cv.take<int>((size_t)(E > 0 ? E : 1) * (k2dLayers + 1));
   b.xyw = cv.take<float4>(n);
   b.rhoKey = cv.take<uint32_t>(n);
+  b.lab = cv.take<int2>(n);
   b.reachD = cv.take<int4>((size_t)(nBlocks > 0 ? nBlocks : 1) * kTilesPerSort);
   b.reachN = cv.take<int4>((size_t)(nBlocks > 0 ? nBlocks : 1) * kTilesPerSort);
   b.hist = cv.take<int>((size_t)kRadix * (nBlocks > 0 ? nBlocks : 1));
@@ -178,6 +219,7 @@ size_t clue2d_workspace_bytes(int64_t N, int nEvents, int nBlocks) {
   add((size_t)(nEvents > 0 ? nEvents : 1) * (k2dLayers + 1) * 4);  // segB
   add(n * 16);                                              // xyw
   add(n * 4);                                               // rhoKey
+  add(n * 8);                                               // lab
   for (int k = 0; k < 2; ++k) add((size_t)(nBlocks > 0 ? nBlocks : 1) * kTilesPerSort * sizeof(int4));  // reach
   add((size_t)kRadix * (size_t)(nBlocks > 0 ? nBlocks : 1) * 4);  // hist
   const int64_t maxScan = (int64_t)kRadix * nBlocks > N ? (int64_t)kRadix * nBlocks : N;

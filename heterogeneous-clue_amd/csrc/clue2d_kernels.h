@@ -885,11 +885,15 @@ __device__ __forceinline__ float density_run(int p, float xi, float yi, int s, i
   return rho;
 }
 
-template <int NT, int ITEMS, int STAGE>
+// rho of every point: rhoKey[p] at its sorted position p (what the nearest-higher step reads) and
+// rho itself, with RHO_OUT at the point's index, else (the index-pass output form) as lab[p].y, from
+// where the output pass copies it to the index in index order.
+template <int NT, int ITEMS, int STAGE, bool RHO_OUT>
 __global__ __launch_bounds__(NT) CLUE_WAVES_PER_EU(kWavesPerEu2D<NT, ITEMS, STAGE>) void k_density2d(const BlockDesc* __restrict__ blocks,
                                                         const float4* __restrict__ xyw, BinTables2D tabs,
                                                         const int4* __restrict__ reach, float dc, float dc2,
-                                                        uint32_t* __restrict__ rhoKey, float* __restrict__ rhoOut) {
+                                                        uint32_t* __restrict__ rhoKey, float* __restrict__ rhoOut,
+                                                        int2* __restrict__ lab) {
   using Shape = CompShape<NT, ITEMS>;
   static_assert(STAGE <= kStageMax, "staged byte offsets take 16 bits");
   __shared__ float4 s_st[STAGE + 1];  // + the spare element a masked second candidate may read
@@ -948,9 +952,11 @@ __global__ __launch_bounds__(NT) CLUE_WAVES_PER_EU(kWavesPerEu2D<NT, ITEMS, STAG
           rho = density_run<false>(p, xi, yi, s, t, xyw, s_st, rlo, dc2, rho);
         }
       }
-      const int i = __float_as_int(me[r].w);
       rhoKey[p] = rho_key(rho);
-      rhoOut[i] = rho;
+      if constexpr (RHO_OUT)
+        rhoOut[__float_as_int(me[r].w)] = rho;
+      else
+        lab[p].y = __float_as_int(rho);
     }
   }
 }
@@ -977,12 +983,12 @@ __device__ __forceinline__ void nh_run(uint64_t qkey, float xi, float yi, int s,
     const bool upd = (cand_key(c) > qkey) & (d2 < best2);
     prev2 = upd ? best2 : prev2;
     best2 = upd ? d2 : best2;
-    best = upd ? __float_as_int(c.z) : best;
+    best = upd ? q : best;
   }
 }
 
 // Candidates q0 and (if two) q1 = q0 + 1, in that order, of the deferred nearest-higher scan; see
-// nh_run.
+// nh_run. `best` is the sorted position of the running minimum.
 template <bool STAGED>
 struct NhStep {
   uint64_t qkey;
@@ -992,7 +998,7 @@ struct NhStep {
   int lo;
   float best2, prev2;
   int best;
-  __device__ __forceinline__ void take(const float4& c, bool valid) {
+  __device__ __forceinline__ void take(const float4& c, int q, bool valid) {
     const float dx = xi - c.x;
     const float dy = yi - c.y;
     const float d2 = dx * dx + dy * dy;
@@ -1000,37 +1006,38 @@ struct NhStep {
     const bool upd = valid & (cand_key(c) > qkey) & (d2 < best2);
     prev2 = upd ? best2 : prev2;
     best2 = upd ? d2 : best2;
-    best = upd ? __float_as_int(c.z) : best;
+    best = upd ? q : best;
   }
   __device__ __forceinline__ void operator()(int q0, int q1, bool two) {
     const float4 c0 = cand<STAGED>(g, st, q0, lo);
     const float4 c1 = cand<STAGED>(g, st, q1, lo);
-    take(c0, true);
-    take(c1, two);
+    take(c0, q0, true);
+    take(c1, q1, two);
   }
 };
 
-// NhStep over staged candidates at byte offsets qa and (if two) qa + 16 (see walk_staged).
+// NhStep over staged candidates at byte offsets qa and (if two) qa + 16 (see walk_staged); `best` is
+// the staged byte offset of the running minimum.
 struct NhStaged {
   uint64_t qkey;
   float xi, yi;
   const unsigned char* st;
   float best2, prev2;
   int best;
-  __device__ __forceinline__ void take(const float4& c, bool valid) {
+  __device__ __forceinline__ void take(const float4& c, int qa, bool valid) {
     const float dx = xi - c.x;
     const float dy = yi - c.y;
     const float d2 = dx * dx + dy * dy;
     const bool upd = valid & (cand_key(c) > qkey) & (d2 < best2);
     prev2 = upd ? best2 : prev2;
     best2 = upd ? d2 : best2;
-    best = upd ? __float_as_int(c.z) : best;
+    best = upd ? qa : best;
   }
   __device__ __forceinline__ void operator()(int qa, bool two) {
     const float4 c0 = staged_at(st, qa);
     const float4 c1 = staged_at(st, qa + 16);
-    take(c0, true);
-    take(c1, two);
+    take(c0, qa, true);
+    take(c1, qa + 16, two);
   }
 };
 
@@ -1046,12 +1053,13 @@ __device__ __forceinline__ void nh_run_exact(uint64_t qkey, float xi, float yi, 
       const float d = sqrtf(d2);
       if (d < delta) {
         delta = d;
-        nh = __float_as_int(c.z);
+        nh = q;
       }
     }
   }
 }
 
+// The nearest higher of a point as its sorted position (-1 if it has none), with delta.
 template <bool STAGED>
 __device__ __forceinline__ int nearest_higher_of(uint64_t qkey, float xi, float yi, const BinTable2D& T,
                                                  const Runs& runs, const NhSrc& src, const float4* st, int lo,
@@ -1068,7 +1076,7 @@ __device__ __forceinline__ int nearest_higher_of(uint64_t qkey, float xi, float 
     }
     best2 = f.best2;
     prev2 = f.prev2;
-    best = f.best;
+    best = f.best >= 0 ? lo + (f.best >> 4) : -1;
   } else {
     NhStep<STAGED> f{qkey, xi, yi, src, st, lo, dm2up, INFINITY, -1};
     walk_runs(runs, f);
@@ -1097,8 +1105,17 @@ __device__ __forceinline__ int nearest_higher_of(uint64_t qkey, float xi, float 
 
 // delta / nearestHigher plus the seed/outlier decision of kernel_findAndAssign_clusters
 // (CLUEAlgoKernels.h:108-137); each seed also sets its bit in the batch's seed words (cleared before
-// the launch), from which the labels step ranks the seeds.
-template <int NT, int ITEMS, int STAGE>
+// the launch), from which the labels step ranks the seeds. Two output forms:
+//  * INDEX_PASS = false: delta, nearestHigher and isSeed at the point's index, and `parent` (index
+//    space: the nearest higher of a follower, the point itself for a root);
+//  * INDEX_PASS = true: one 16-byte record per point at its index, rec[i] = {delta bits,
+//    nearestHigher, isSeed, the point's sorted position}, and `parent` in tile order, written
+//    coalesced: a follower's link is the sorted position of its nearest higher, a root's is ~i (its
+//    own index, negative). The labels step then walks the links in tile order (k_labels_tile2d) and
+//    writes every output in index order (k_outputs2d). One scattered 16-byte write per point instead
+//    of four 4-byte ones: where hit indices are unrelated to position, each scattered write costs a
+//    memory burst.
+template <int NT, int ITEMS, int STAGE, bool INDEX_PASS>
 __global__ __launch_bounds__(NT) CLUE_WAVES_PER_EU(kWavesPerEu2D<NT, ITEMS, STAGE>) void k_nh2d(const BlockDesc* __restrict__ blocks,
                                                    const float4* __restrict__ xyw, const uint32_t* __restrict__ rhoKey,
                                                    BinTables2D tabs,
@@ -1106,7 +1123,8 @@ __global__ __launch_bounds__(NT) CLUE_WAVES_PER_EU(kWavesPerEu2D<NT, ITEMS, STAG
                                                    float dc, float rhoc,
                                                    float odf, float dm2, float dm2up, float* __restrict__ deltaOut,
                                                    int32_t* __restrict__ nhOut, int32_t* __restrict__ isSeedOut,
-                                                   int* __restrict__ parent, uint64_t* __restrict__ seedWord) {
+                                                   int* __restrict__ parent, int4* __restrict__ rec,
+                                                   uint64_t* __restrict__ seedWord) {
   using Shape = CompShape<NT, ITEMS>;
   static_assert(STAGE <= kStageMax, "staged byte offsets take 16 bits");
   __shared__ float4 s_st[STAGE + 1];  // + the spare element a masked second candidate may read
@@ -1151,19 +1169,93 @@ __global__ __launch_bounds__(NT) CLUE_WAVES_PER_EU(kWavesPerEu2D<NT, ITEMS, STAG
       const float rhoi = rho_of_key(rk);
       const uint64_t qkey = query_key(rk, i);
       float delta;
-      const int nhg = staged ? nearest_higher_of<true>(qkey, me[r].x, me[r].y, T, runs[r], src, s_st, rlo, dm2, dm2up,
+      const int nhp = staged ? nearest_higher_of<true>(qkey, me[r].x, me[r].y, T, runs[r], src, s_st, rlo, dm2, dm2up,
                                                        delta)
                              : nearest_higher_of<false>(qkey, me[r].x, me[r].y, T, runs[r], src, s_st, rlo, dm2,
                                                         dm2up, delta);
-      deltaOut[i] = delta;
-      nhOut[i] = nhg >= 0 ? nhg - evBase : -1;
+      // the nearest higher's point index, from the staged candidate or its sorted record
+      const int nhg = nhp < 0 ? -1 : staged ? __float_as_int(s_st[nhp - rlo].z) : __float_as_int(xyw[nhp].w);
       const bool seed = (delta > dc) && (rhoi >= rhoc);
       const bool outlier = (delta > dm) && (rhoi < rhoc);
-      isSeedOut[i] = seed ? 1 : 0;
+      const bool root = seed || outlier || nhg < 0;
       if (seed) atomicOr(reinterpret_cast<unsigned long long*>(seedWord + (i >> 6)), 1ull << (i & 63));
       // Followers point at their nearest higher; seeds, outliers (and a point without a nearest
       // higher, which the reference cannot register) are roots.
-      parent[i] = (seed || outlier || nhg < 0) ? i : nhg;
+      if constexpr (INDEX_PASS) {
+        rec[i] = make_int4(__float_as_int(delta), nhg >= 0 ? nhg - evBase : -1, seed ? 1 : 0, p);
+        parent[p] = root ? ~i : nhp;
+      } else {
+        deltaOut[i] = delta;
+        nhOut[i] = nhg >= 0 ? nhg - evBase : -1;
+        isSeedOut[i] = seed ? 1 : 0;
+        parent[i] = root ? i : nhg;
+      }
+    }
+  }
+}
+
+// Labels of the index-pass output form, in tile order: each sorted position walks the links
+// (parentS, see k_nh2d) until it reads a root's ~index; the lanes of a wave start at consecutive
+// sorted positions, whose links lie in the same few search boxes, so their loads share cache lines.
+// Writes lab[p].x = the label of p (lab[p].y holds its rho: the pair the output pass gathers with
+// one load).
+__global__ __launch_bounds__(kThreads) void k_labels_tile2d(const BlockDesc* __restrict__ blocks,
+                                                            const int* __restrict__ parentS, SeedRank sr,
+                                                            const int* __restrict__ evOff, int2* __restrict__ lab) {
+  const BlockDesc bd = blocks[xcd_block((int)blockIdx.x, (int)gridDim.x)];
+  const int rank0 = rank_of(sr, evOff[bd.event]);
+  int v[kSortItems];
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int p = bd.begin + r * kThreads + (int)threadIdx.x;
+    v[r] = p < bd.end ? parentS[p] : -1;
+  }
+  // every link leads to a strictly higher point (or is a root's ~index), so each walk ends
+  bool any = true;
+  while (any) {
+    any = false;
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) v[r] = v[r] >= 0 ? parentS[v[r]] : v[r];
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) any |= v[r] >= 0;
+  }
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int p = bd.begin + r * kThreads + (int)threadIdx.x;
+    if (p < bd.end) {
+      const int root = ~v[r];
+      lab[p].x = seed_of(sr, root) ? rank_of(sr, root) - rank0 : -1;
+    }
+  }
+}
+
+// The outputs of the index-pass form in index order: rec[i] = {delta bits, nearestHigher, isSeed,
+// sorted position p of i} (coalesced) and lab[p] (the one scattered load per point) give rho,
+// delta, nearestHigher, isSeed and the label of point i, all written coalesced.
+__global__ __launch_bounds__(kThreads) void k_outputs2d(const BlockDesc* __restrict__ blocks,
+                                                        const int4* __restrict__ rec, const int2* __restrict__ lab,
+                                                        float* __restrict__ rhoOut, float* __restrict__ deltaOut,
+                                                        int32_t* __restrict__ nhOut, int32_t* __restrict__ isSeedOut,
+                                                        int32_t* __restrict__ label) {
+  const BlockDesc bd = blocks[xcd_block((int)blockIdx.x, (int)gridDim.x)];
+  int4 v[kSortItems];
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int i = bd.begin + r * kThreads + (int)threadIdx.x;
+    v[r] = rec[i < bd.end ? i : bd.begin];
+  }
+  int2 l[kSortItems];
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) l[r] = lab[v[r].w];
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int i = bd.begin + r * kThreads + (int)threadIdx.x;
+    if (i < bd.end) {
+      rhoOut[i] = __int_as_float(l[r].y);
+      deltaOut[i] = __int_as_float(v[r].x);
+      nhOut[i] = v[r].y;
+      isSeedOut[i] = v[r].z;
+      label[i] = l[r].x;
     }
   }
 }

@@ -272,12 +272,23 @@ inline SeedRank seed_rank_carve(Carver& cv, int64_t N) {
   return r;
 }
 
+// Seed bit and seed rank (seeds of the batch before point i, in index order) of point i.
+__device__ __forceinline__ bool seed_of(const SeedRank& sr, int i) {
+  return (sr.word[i >> 6] >> (i & 63)) & 1ull;
+}
+__device__ __forceinline__ int rank_of(const SeedRank& sr, int i) {
+  return sr.tilePre[i / kSeedTile] + sr.wordPre[i >> 6] + __popcll(sr.word[i >> 6] & ((1ull << (i & 63)) - 1ull));
+}
+
 // clusterIndex[i] = isSeed[root(i)] ? (#seeds of the event before root) : -1, where root(i) is
 // reached by following parent[] (parent[i] == i at roots). `part` needs scan_partials(seed_tiles(N))
 // ints. isSeed == nullptr: the seed words are already set (cleared by clear_seed_words, then one bit
 // per seed set by the caller's kernels).
 void launch_labels(const BlockDesc* blocksDev, int nBlocks, const int* parent, const int* isSeed, int64_t N,
                    const int* evOffDev, int32_t* label, const SeedRank& sr, int* part, hipStream_t s);
+// The seed words' prefixes and scanned tile counts alone (the words already set), for callers with
+// their own labels kernel.
+void launch_seed_ranks(int64_t N, const SeedRank& sr, int* part, hipStream_t s);
 // Zero the seed words of N points on the stream (a fill kernel, no memset node in a captured graph).
 void clear_seed_words(const SeedRank& sr, int64_t N, hipStream_t s);
 

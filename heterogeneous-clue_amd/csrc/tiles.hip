@@ -46,16 +46,28 @@ void build_table(const uint32_t* keySorted, const BlockDesc* blocksDev, int nBlo
                                                stride, table);
 }
 
-void launch_labels(const BlockDesc* blocksDev, int nBlocks, const int* parent, const int* isSeed, int64_t N,
-                   const int* evOffDev, int32_t* label, const SeedRank& sr, int* part, hipStream_t s) {
-  if (nBlocks <= 0 || N <= 0) return;
+namespace {
+// Seed words (from isSeed, or already set when isSeed is null), word prefixes and scanned tile counts.
+void seed_ranks(const int* isSeed, int64_t N, const SeedRank& sr, int* part, hipStream_t s) {
   const int tiles = seed_tiles(N);
   if (isSeed)
     k_seed_bits<<<tiles, kThreads, 0, s>>>(isSeed, N, sr);
   else
     k_seed_prefix<<<(tiles * 32 + kThreads - 1) / kThreads, kThreads, 0, s>>>((int64_t)tiles * 32, sr);
   scan_exclusive(sr.tilePre, sr.tilePre, tiles, part, s);
+}
+}  // namespace
+
+void launch_labels(const BlockDesc* blocksDev, int nBlocks, const int* parent, const int* isSeed, int64_t N,
+                   const int* evOffDev, int32_t* label, const SeedRank& sr, int* part, hipStream_t s) {
+  if (nBlocks <= 0 || N <= 0) return;
+  seed_ranks(isSeed, N, sr, part, s);
   k_labels<<<nBlocks, kThreads, 0, s>>>(blocksDev, parent, sr, evOffDev, label);
+}
+
+void launch_seed_ranks(int64_t N, const SeedRank& sr, int* part, hipStream_t s) {
+  if (N <= 0) return;
+  seed_ranks(nullptr, N, sr, part, s);
 }
 
 void clear_seed_words(const SeedRank& sr, int64_t N, hipStream_t s) {

@@ -226,12 +226,6 @@ __global__ __launch_bounds__(kThreads) void k_seed_prefix(int64_t nWords, SeedRa
   }
 }
 
-__device__ __forceinline__ bool seed_of(const SeedRank& sr, int i) {
-  return (sr.word[i >> 6] >> (i & 63)) & 1ull;
-}
-__device__ __forceinline__ int rank_of(const SeedRank& sr, int i) {
-  return sr.tilePre[i / kSeedTile] + sr.wordPre[i >> 6] + __popcll(sr.word[i >> 6] & ((1ull << (i & 63)) - 1ull));
-}
 
 __global__ __launch_bounds__(kThreads) void k_labels(const BlockDesc* __restrict__ blocks,
                                                      const int* __restrict__ parent,

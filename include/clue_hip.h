@@ -186,6 +186,14 @@ int32_t clue2d_run_batch_device(clue_ctx* ctx, const clue2d_params* p, int32_t n
  * choice). All variants give the same bits. No reference counterpart. */
 #define CLUE_PLAN_GRAPH 1
 #define CLUE_PLAN_AUTOTUNE 2
+/* CLUE2D plans: CLUE_PLAN_INDEX_PASS selects the second output form of the nearest-higher step. Its
+ * kernel writes one 16-byte record per point at the point's index instead of delta, nearestHigher and
+ * isSeed separately, keeps the follower links in tile order, and the labels step writes the three
+ * outputs in index order while it walks the links. That pays when hit indices are unrelated to hit
+ * positions (every scattered 4-byte write costs a memory burst); hits that arrive grouped by
+ * position are served better by the direct writes. With CLUE_PLAN_AUTOTUNE the first launch times
+ * both forms and keeps the faster (clue_plan_index_pass reports it). Both forms give the same bits. */
+#define CLUE_PLAN_INDEX_PASS 4
 clue_plan* clue2d_plan_create(clue_ctx* ctx, const clue2d_params* p, int32_t n_events, const int64_t* offsets,
                               const float* x, const float* y, const int32_t* layer, const float* weight, float* rho,
                               float* delta, int32_t* nearestHigher, int32_t* isSeed, int32_t* clusterIndex,
@@ -230,6 +238,9 @@ int32_t clue_plan_launch(clue_plan* plan, void* stream);
 /* CLUE2D plan: the path it runs (CLUE_PATH_FUSED or CLUE_PATH_GLOBAL), or -1 before the first launch
  * has decided it; CLUE3D plans: CLUE_PATH_GLOBAL. */
 int32_t clue_plan_path(const clue_plan* plan);
+/* 1 when a CLUE2D plan uses the index-pass output form (CLUE_PLAN_INDEX_PASS, or the autotuner's
+ * choice after an autotuned plan's first launch), else 0. No reference counterpart. */
+int32_t clue_plan_index_pass(const clue_plan* plan);
 /* Points per workgroup and LDS staging capacity (points) of the density and the
  * nearest-higher kernels it launches, as out[0..3] = {density points, density stage, nh points,
  * nh stage}. Before an autotuned plan's first launch these are the defaults. */

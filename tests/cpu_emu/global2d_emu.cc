@@ -1,8 +1,8 @@
 // Runs the CLUE2D global-sort path (layer pass, per-layer bin sort with the bin table, density,
 // nearest higher, seed ranks, labels) on the CPU through the HIP-model emulation, with clue2d_enqueue's launch sequence
 // and grids. Test infrastructure only.
-// usage: global2d_emu <in> <out> [density variant] [nearest-higher variant] (formats: launch_emu.h;
-// variants index kVariants2D)
+// usage: global2d_emu <in> <out> [density variant] [nearest-higher variant] [index pass 0/1]
+// (formats: launch_emu.h; variants index kVariants2D; the last selects k_nh2d's output form)
 #include <type_traits>
 
 #include "clue2d_kernels.h"
@@ -86,22 +86,44 @@ int main(int argc, char** argv) {
       k_reach2d(blocks.data(), nTiles, keyS.data(), tabs, dc, odf, reachD.data(), reachN.data());
     });
     const int dv = argc > 3 ? atoi(argv[3]) : 0, nv = argc > 4 ? atoi(argv[4]) : 0;
+    const bool indexPass = argc > 5 && atoi(argv[5]) != 0;
+    std::vector<int2> lab(N, int2{P, P});
     dispatch(dv, [&](auto T, auto I, auto S) {
       emu::launch(CompShape<T, I>::kPerSort * nb, T, [&] {
-        k_density2d<T, I, S>(blocks.data(), xyw.data(), tabs, reachD.data(), dc, dc2,
-                             rhoKey.data(), rho.data());
+        if (indexPass)
+          k_density2d<T, I, S, false>(blocks.data(), xyw.data(), tabs, reachD.data(), dc, dc2, rhoKey.data(),
+                                      rho.data(), lab.data());
+        else
+          k_density2d<T, I, S, true>(blocks.data(), xyw.data(), tabs, reachD.data(), dc, dc2, rhoKey.data(),
+                                     rho.data(), lab.data());
       });
     });
-    int* parent = reinterpret_cast<int*>(lg.data());
+    // the links and records reuse the layer-order records and the bin sort's slots, as clue2d.hip does
+    int4* rec = reinterpret_cast<int4*>(lg.data());
+    int* parent = indexPass ? scr.data() : reinterpret_cast<int*>(lg.data());
     SeedRankHost seeds(N);
     std::fill(seeds.word.begin(), seeds.word.end(), 0ull);  // clear_seed_words
     dispatch(nv, [&](auto T, auto I, auto S) {
       emu::launch(CompShape<T, I>::kPerSort * nb, T, [&] {
-        k_nh2d<T, I, S>(blocks.data(), xyw.data(), rhoKey.data(), tabs, reachN.data(), evOff.data(), dc, rhoc, odf,
-                        dm2, dm2up, delta.data(), nh.data(), isSeed.data(), parent, seeds.word.data());
+        if (indexPass)
+          k_nh2d<T, I, S, true>(blocks.data(), xyw.data(), rhoKey.data(), tabs, reachN.data(), evOff.data(), dc, rhoc,
+                                odf, dm2, dm2up, delta.data(), nh.data(), isSeed.data(), parent, rec,
+                                seeds.word.data());
+        else
+          k_nh2d<T, I, S, false>(blocks.data(), xyw.data(), rhoKey.data(), tabs, reachN.data(), evOff.data(), dc, rhoc,
+                                 odf, dm2, dm2up, delta.data(), nh.data(), isSeed.data(), parent, rec,
+                                 seeds.word.data());
       });
     });
-    launch_labels(blocks.data(), nb, parent, nullptr, N, evOff.data(), ci.data(), seeds.view(), part.data());
+    if (indexPass) {
+      launch_seed_ranks(N, seeds.view(), part.data());
+      emu::launch(nb, kThreads, [&] { k_labels_tile2d(blocks.data(), parent, seeds.view(), evOff.data(), lab.data()); });
+      emu::launch(nb, kThreads, [&] {
+        k_outputs2d(blocks.data(), rec, lab.data(), rho.data(), delta.data(), nh.data(), isSeed.data(), ci.data());
+      });
+    } else {
+      launch_labels(blocks.data(), nb, parent, nullptr, N, evOff.data(), ci.data(), seeds.view(), part.data());
+    }
   }
   write_output(argv[2], err, 0, rho, delta, nh, isSeed, ci);
   return 0;

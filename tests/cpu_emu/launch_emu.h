@@ -49,6 +49,13 @@ inline void launch_labels(const BlockDesc* blocks, int nBlocks, const int* paren
   emu::launch(nBlocks, kThreads, [&] { k_labels(blocks, parent, sr, evOff, label); });
 }
 
+inline void launch_seed_ranks(int64_t N, const SeedRank& sr, int* part) {
+  if (N <= 0) return;
+  const int tiles = seed_tiles(N);
+  emu::launch((tiles * 32 + kThreads - 1) / kThreads, kThreads, [&] { k_seed_prefix((int64_t)tiles * 32, sr); });
+  scan_exclusive(sr.tilePre, sr.tilePre, tiles, part);
+}
+
 // Seed-rank buffers of N points, poisoned like fresh device memory.
 struct SeedRankHost {
   std::vector<uint64_t> word;

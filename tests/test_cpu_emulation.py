@@ -178,6 +178,37 @@ def test_emulated_global_variants(emulator, tmp_path, variants):
     helpers.assert_bit_exact(got, want, helpers.OUT2D, f"emulated variants {variants}")
 
 
+@pytest.mark.parametrize("exe", ["global2d_emu", "global2d_emu_ldsz", "global2d_emu_tsan"])
+@pytest.mark.parametrize("variants", [(0, 0), (1, 1), (4, 5)])
+def test_emulated_index_pass(emulator, tmp_path, oracle_lib, exe, variants):
+    """The nearest-higher step's index-pass output form (one index-order record per point, tile-order
+    follower links, outputs copied to the points' indices by the labels step) gives the oracle's bits:
+    a three-event batch with an empty event and hits in random order, and the layer-ordered golden
+    'bin_edges' case, under AddressSanitizer, with LDS redzones and under ThreadSanitizer."""
+    evs = [synth.clue2d_event(n, 90 + i, shuffle=True) for i, n in enumerate([3000, 0, 900])]
+    packed, offs = synth.pack(evs)
+    err, _, got = run_emulated(EMU / "_build" / exe, tmp_path, packed, offs, extra=variants + (1,))
+    assert err == 0
+    helpers.assert_bit_exact(got, oracle_lib.clue2d_batch(packed, offs), helpers.OUT2D, f"index pass {exe}")
+    ins, want, extra = helpers.load_case("clue2d", "bin_edges")
+    dc, rhoc, odf = (float(v) for v in extra["params"])
+    err, _, got = run_emulated(EMU / "_build" / exe, tmp_path, ins, np.array([0, ins["x"].size]), dc, rhoc, odf,
+                               extra=variants + (1,))
+    assert err == 0
+    helpers.assert_bit_exact(got, want, helpers.OUT2D, f"index pass bin_edges {exe}")
+
+
+@pytest.mark.parametrize("name", helpers.golden_names("clue2d"))
+def test_emulated_index_pass_golden(emulator, tmp_path, name):
+    """Every CLUE2D golden case through the index-pass output form."""
+    ins, want, extra = helpers.load_case("clue2d", name)
+    dc, rhoc, odf = (float(v) for v in extra["params"])
+    err, _, got = run_emulated(EXE_GLOBAL, tmp_path, ins, np.array([0, ins["x"].size]), dc, rhoc, odf,
+                               extra=(0, 0, 1))
+    assert err == 0
+    helpers.assert_bit_exact(got, want, helpers.OUT2D, f"index pass {name}")
+
+
 @pytest.mark.parametrize("params", [helpers.EXTREME_PARAMS[k] for k in (4, 5)])
 def test_emulated_global_extremes(emulator, tmp_path, oracle_lib, params):
     """NaN, infinite and out-of-grid coordinates with a NaN and a very large distance:

@@ -83,7 +83,8 @@ def test_autotuned_plan(dev, oracle_lib):
     din = dev.upload_soa(packed, clue_hip.FIELDS_3D_IN)
     dout = dev.alloc_soa(N, clue_hip.FIELDS_3D_OUT)
     plan = dev.clue3d_plan(din, dout, offs, clue_hip.params3d(S=2), use_graph=True, autotune=True)
-    assert plan.tuning == {"density_points": 512, "density_stage": 2048, "nh_points": 512, "nh_stage": 2048}
+    assert plan.tuning == {"density_points": 512, "density_stage": 2048, "nh_points": 512, "nh_stage": 2048,
+                           "nh_index_pass": False}
     for rep in range(3):
         for a in dout.values():
             a.upload(np.full(N, -7, a.dtype))

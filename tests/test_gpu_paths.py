@@ -159,7 +159,8 @@ def test_autotuned_plan(dev, oracle_lib):
     din = dev.upload_soa(packed, clue_hip.FIELDS_2D_IN)
     dout = dev.alloc_soa(N, clue_hip.FIELDS_2D_OUT)
     plan = dev.clue2d_plan(din, dout, offs, use_graph=True, autotune=True)
-    assert plan.tuning == {"density_points": 512, "density_stage": 1024, "nh_points": 512, "nh_stage": 1024}
+    assert plan.tuning == {"density_points": 512, "density_stage": 1024, "nh_points": 512, "nh_stage": 1024,
+                           "nh_index_pass": False}
     for rep in range(3):
         for a in dout.values():
             a.upload(np.full(N, -7, a.dtype))
@@ -174,3 +175,57 @@ def test_autotuned_plan(dev, oracle_lib):
     plan.destroy()
     for a in list(din.values()) + list(dout.values()):
         a.free()
+
+
+def run_plan(dev, packed, offs, want, label, reps=3, **kw):
+    """Plan launches (graph replays after the first) over fresh output poison, each compared bit for bit."""
+    N = int(offs[-1])
+    din = dev.upload_soa(packed, clue_hip.FIELDS_2D_IN)
+    dout = dev.alloc_soa(N, clue_hip.FIELDS_2D_OUT)
+    plan = dev.clue2d_plan(din, dout, offs, **kw)
+    for rep in range(reps):
+        for a in dout.values():
+            a.upload(np.full(N, -7, a.dtype))
+        plan.launch()
+        dev.check()
+        helpers.assert_bit_exact(dev.download_soa(dout), want, helpers.OUT2D, f"{label} launch {rep}")
+    t = plan.tuning
+    plan.destroy()
+    for a in list(din.values()) + list(dout.values()):
+        a.free()
+    return t
+
+
+def test_index_pass_plan(dev, oracle_lib):
+    """The nearest-higher step's index-pass output form (CLUE_PLAN_INDEX_PASS): shuffled and
+    layer-ordered events, an empty one, one with a layer above 65535 points (32-bit table entries)."""
+    evs = [synth.clue2d_event(n, 400 + i, shuffle=i % 2 == 0) for i, n in enumerate([150_000, 2_500, 0, 40_000])]
+    evs.append(layered_event([70_000, 300], 9))
+    packed, offs = synth.pack(evs)
+    print(f"case clue2d index-pass plan n={int(offs[-1])}", flush=True)
+    want = oracle_lib.clue2d_batch(packed, offs)
+    t = run_plan(dev, packed, offs, want, "index pass", use_graph=True, index_pass=True)
+    assert t["nh_index_pass"] is True
+    t = run_plan(dev, packed, offs, want, "direct", reps=1, use_graph=False)
+    assert t["nh_index_pass"] is False
+
+
+def test_index_pass_golden(dev):
+    """Every CLUE2D golden case through an index-pass plan."""
+    for name in helpers.golden_names("clue2d"):
+        ins, want, extra = helpers.load_case("clue2d", name)
+        dc, rhoc, odf = (float(v) for v in extra["params"])
+        print(f"case clue2d index-pass golden {name}", flush=True)
+        run_plan(dev, ins, np.array([0, ins["x"].size]), want, f"index pass {name}", reps=2,
+                 p=clue_hip.params2d(dc, rhoc, odf), use_graph=True, index_pass=True)
+
+
+def test_autotuned_plan_shuffled(dev, oracle_lib):
+    """An autotuned plan on events whose hits arrive in random order: whichever output form the tuner
+    keeps, the tuning launch and the replays are bit-exact."""
+    sizes = [300_000, 5_000, 80_000]
+    packed, offs = synth.pack([synth.clue2d_event(n, 500 + i, shuffle=True) for i, n in enumerate(sizes)])
+    print(f"case clue2d autotuned shuffled plan n={int(offs[-1])}", flush=True)
+    want = oracle_lib.clue2d_batch(packed, offs)
+    t = run_plan(dev, packed, offs, want, "autotuned shuffled", use_graph=True, autotune=True)
+    print(f"tuning {t}", flush=True)
