@@ -1202,20 +1202,36 @@ __global__ __launch_bounds__(NT) CLUE_WAVES_PER_EU(kWavesPerEu2D<NT, ITEMS, STAG
 __global__ __launch_bounds__(kThreads) void k_labels_tile2d(const BlockDesc* __restrict__ blocks,
                                                             const int* __restrict__ parentS, SeedRank sr,
                                                             const int* __restrict__ evOff, int2* __restrict__ lab) {
+  // the block's own links, staged: links inside the block's range are read from LDS
+  __shared__ int s_par[kSortBlock];
   const BlockDesc bd = blocks[xcd_block((int)blockIdx.x, (int)gridDim.x)];
   const int rank0 = rank_of(sr, evOff[bd.event]);
+  const int n = bd.end - bd.begin;
   int v[kSortItems];
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    const int p = bd.begin + r * kThreads + (int)threadIdx.x;
-    v[r] = p < bd.end ? parentS[p] : -1;
+    const int k = r * kThreads + (int)threadIdx.x;
+    v[r] = k < n ? parentS[bd.begin + k] : -1;
   }
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int k = r * kThreads + (int)threadIdx.x;
+    if (k < n) s_par[k] = v[r];
+  }
+  __syncthreads();
   // every link leads to a strictly higher point (or is a root's ~index), so each walk ends
   bool any = true;
   while (any) {
     any = false;
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) v[r] = v[r] >= 0 ? parentS[v[r]] : v[r];
+    for (int r = 0; r < kSortItems; ++r) {
+      const int k = v[r] - bd.begin;
+      const bool local = (unsigned)k < (unsigned)n;
+      int a = v[r], g = v[r];
+      if (v[r] >= 0 && local) a = s_par[k];
+      if (v[r] >= 0 && !local) g = parentS[v[r]];
+      v[r] = local ? a : g;
+    }
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) any |= v[r] >= 0;
   }

@@ -196,6 +196,13 @@ def test_emulated_index_pass(emulator, tmp_path, oracle_lib, exe, variants):
                                extra=variants + (1,))
     assert err == 0
     helpers.assert_bit_exact(got, want, helpers.OUT2D, f"index pass bin_edges {exe}")
+    ev = helpers.extreme_event(n_normal=200)
+    for dc, rhoc, odf in (helpers.EXTREME_PARAMS[4], helpers.EXTREME_PARAMS[5]):
+        err, _, got = run_emulated(EMU / "_build" / exe, tmp_path, ev, np.array([0, ev["x"].size]), dc, rhoc, odf,
+                                   extra=variants + (1,))
+        assert err == 0
+        want = oracle_lib.clue2d(ev["x"], ev["y"], ev["layer"], ev["weight"], dc, rhoc, odf)
+        helpers.assert_bit_exact(got, want, helpers.OUT2D, f"index pass extremes {exe} {dc}")
 
 
 @pytest.mark.parametrize("name", helpers.golden_names("clue2d"))

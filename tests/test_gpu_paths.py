@@ -229,3 +229,16 @@ def test_autotuned_plan_shuffled(dev, oracle_lib):
     want = oracle_lib.clue2d_batch(packed, offs)
     t = run_plan(dev, packed, offs, want, "autotuned shuffled", use_graph=True, autotune=True)
     print(f"tuning {t}", flush=True)
+
+
+@pytest.mark.parametrize("params", helpers.EXTREME_PARAMS)
+def test_index_pass_extremes(dev, oracle_lib, params):
+    """NaN, infinite and out-of-grid coordinates with every extreme distance set, batched beside an
+    ordinary shuffled event, through the index-pass output form."""
+    dc, rhoc, odf = params
+    packed, offs = synth.pack([helpers.extreme_event(), synth.clue2d_event(20000, 3, shuffle=True),
+                               helpers.extreme_event(seed=6)])
+    print(f"case clue2d index-pass extremes {params} n={int(offs[-1])}", flush=True)
+    want = oracle_lib.clue2d_batch(packed, offs, dc, rhoc, odf)
+    run_plan(dev, packed, offs, want, f"index pass extremes {params}", reps=2, p=clue_hip.params2d(dc, rhoc, odf),
+             use_graph=True, index_pass=True)
