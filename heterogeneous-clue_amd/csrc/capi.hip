@@ -270,6 +270,7 @@ int plan_setup_global(clue_plan* pl) {
   int rc = ensure(c, pl->ws, clue2d_workspace_bytes(N, E, nb));
   if (rc) return rc;
   clue2d_carve(pl->ws.base, N, E, nb, pl->b2);
+  clue2d_segment_bounds(pl->b2, pl->offsets.data(), E);
   pl->b2.err = c->dErr;
   pl->b2.nhIndexPass = pl->indexPass ? 1 : 0;
   std::vector<BlockDesc> blocks;
@@ -407,6 +408,7 @@ int run2d(clue_ctx* c, const Clue2DParams& p, int E, const int64_t* offsets, con
   if (rc) return rc;
   Clue2DBuffers b;
   clue2d_carve(c->scratch2d.base, N, E, nb, b);
+  clue2d_segment_bounds(b, offsets, E);
   b.err = c->dErr;
   if (single) {
     rc = single_setup(c, (int)N, nb, b.blocks, b.evOff, s);
@@ -692,6 +694,8 @@ int32_t clue2d_run_device_noalloc(clue_ctx* c, const clue2d_params* p, int32_t n
   // need no clearing.
   Clue2DBuffers b;
   clue2d_carve(base, n, 1, nb, b);
+  const int64_t offs[2] = {0, n};
+  clue2d_segment_bounds(b, offs, 1);
   b.err = c->dErr;
   int rc = single_setup(c, n, nb, b.blocks, b.evOff, s);
   if (rc) return rc;

@@ -37,6 +37,11 @@ struct Clue2DBuffers {
   float4* lg = nullptr;      // layer order: {x, y, weight, point index as int bits}; later the parent
                              // array, or the index-order records of the index-pass output form
   int* segB = nullptr;       // per event, the first position of each layer (E * (k2dLayers + 1))
+  int* segList = nullptr;    // the layers of the largest bin-sort class (up to E * k2dLayers)
+  int* segCnt = nullptr;     // their count
+  // Workgroups launched for the largest class: at most that many layers can be in it (from the
+  // event sizes, clue2d_segment_bounds), or -1 for E * k2dLayers.
+  int segBoundH = -1;
   float4* xyw = nullptr;   // tile order: {x, y, weight, point index as int bits}
   uint32_t* rhoKey = nullptr;  // tile order: rho as an order-preserving unsigned key (rho_key)
   int2* lab = nullptr;         // tile order, index-pass output form: {label, rho bits}
@@ -87,6 +92,10 @@ constexpr int kClue2DVariants = 6;
 // Points per block and staged points of a variant.
 void clue2d_variant_shape(int variant, int& points, int& stage);
 size_t clue2d_workspace_bytes(int64_t N, int nEvents, int nBlocks);
+// Tight launch size of the largest bin-sort class for a batch with these offsets (E + 1 entries):
+// an event of n points has at most n / 10241 layers above 10240 points. Call after clue2d_carve
+// (which resets it to the whole grid).
+void clue2d_segment_bounds(Clue2DBuffers& b, const int64_t* offsets, int E);
 void clue2d_carve(void* base, int64_t N, int E, int nBlocks, Clue2DBuffers& b);
 
 }  // namespace clue

@@ -14,6 +14,8 @@
 #include "clue2d_kernels.h"
 
 #include <float.h>
+
+#include <algorithm>
 #include <math.h>
 #include <string.h>
 
@@ -143,14 +145,18 @@ void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DPara
   scan_exclusive(b.hist, b.hist, nHist, b.part, s);
   k_radix_scatter<LayerOf2D, DigitLayer2D, EmitSeg2D><<<b.nBlocks, kThreads, 0, s>>>(
       b.blocks, layerOf, b.hist, EmitSeg2D{io.x, io.y, io.weight, b.lg});
-  k_layer_starts2d<<<b.nBlocks, kRadix, 0, s>>>(b.blocks, b.hist, b.segB);
   const int nSeg = b.E * k2dLayers;
-  k_segsort2d<kSegThreadsS, 0, kSegSmall><<<nSeg, kSegThreadsS, 0, s>>>(b.evOff, b.segB, b.lg, b.scr, b.keyS, b.xyw, b.table16,
-                                                      b.table, b.tableStride);
-  k_segsort2d<kSegThreadsL, kSegSmall, kSegCap><<<nSeg, kSegThreadsL, 0, s>>>(b.evOff, b.segB, b.lg, b.scr, b.keyS, b.xyw, b.table16,
-                                                             b.table, b.tableStride);
-  k_segsort2d<kSegThreadsH, kSegCap, kSegNone><<<nSeg, kSegThreadsH, 0, s>>>(b.evOff, b.segB, b.lg, b.scr, b.keyS, b.xyw, b.table16,
-                                                            b.table, b.tableStride);
+  fill_i32(b.segCnt, 0, 1, s);
+  k_layer_starts2d<<<b.nBlocks, kRadix, 0, s>>>(b.blocks, b.hist, b.segB, b.segList, b.segCnt);
+  k_segsort2d<kSegThreadsS, 0, kSegSmall><<<nSeg, kSegThreadsS, 0, s>>>(
+      b.evOff, b.segB, b.lg, b.scr, b.keyS, b.xyw, b.table16, b.table, b.tableStride, nullptr, nullptr);
+  k_segsort2d<kSegThreadsL, kSegSmall, kSegCap><<<nSeg, kSegThreadsL, 0, s>>>(
+      b.evOff, b.segB, b.lg, b.scr, b.keyS, b.xyw, b.table16, b.table, b.tableStride, nullptr, nullptr);
+  // the largest class: as many workgroups as the event sizes allow such layers (the list holds them)
+  const int gridH = b.segBoundH >= 0 ? b.segBoundH : nSeg;
+  if (gridH > 0)
+    k_segsort2d<kSegThreadsH, kSegCap, kSegNone><<<gridH, kSegThreadsH, 0, s>>>(
+        b.evOff, b.segB, b.lg, b.scr, b.keyS, b.xyw, b.table16, b.table, b.tableStride, b.segList, b.segCnt);
   const BinTables2D tabs{b.table16, b.table, b.segB, b.tableStride};
   const int nTiles = b.nBlocks * kTilesPerSort;
   k_reach2d<<<(nTiles + kReachThreads - 1) / kReachThreads, kReachThreads, 0, s>>>(
@@ -176,6 +182,12 @@ void clue2d_enqueue_step(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2
   }
 }
 
+void clue2d_segment_bounds(Clue2DBuffers& b, const int64_t* offsets, int E) {
+  int64_t g = 0;
+  for (int e = 0; e < E; ++e) g += std::min<int64_t>(k2dLayers, (offsets[e + 1] - offsets[e]) / (kSegCap + 1));
+  b.segBoundH = (int)g;
+}
+
 void clue2d_variant_shape(int variant, int& points, int& stage) {
   const Variant2D v = kVariants2D[variant >= 0 && variant < kNumVariants2D ? variant : 0];
   points = v.threads * v.items;
@@ -193,6 +205,9 @@ void clue2d_carve(void* base, int64_t N, int E, int nBlocks, Clue2DBuffers& b) {
   b.scr = cv.take<int>(n);
   b.lg = cv.take<float4>(n);
   b.segB = cv.take<int>((size_t)(E > 0 ? E : 1) * (k2dLayers + 1));
+  b.segList = cv.take<int>((size_t)(E > 0 ? E : 1) * k2dLayers);
+  b.segCnt = cv.take<int>(1);
+  b.segBoundH = -1;
   b.xyw = cv.take<float4>(n);
   b.rhoKey = cv.take<uint32_t>(n);
   b.lab = cv.take<int2>(n);
@@ -217,6 +232,8 @@ size_t clue2d_workspace_bytes(int64_t N, int nEvents, int nBlocks) {
   add(n * 4);                                               // scr
   add(n * 16);                                              // lg
   add((size_t)(nEvents > 0 ? nEvents : 1) * (k2dLayers + 1) * 4);  // segB
+  add((size_t)(nEvents > 0 ? nEvents : 1) * k2dLayers * 4);        // segList
+  add(4);                                                           // segCnt
   add(n * 16);                                              // xyw
   add(n * 4);                                               // rhoKey
   add(n * 8);                                               // lab

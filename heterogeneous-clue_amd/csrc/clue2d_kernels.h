@@ -66,29 +66,44 @@ struct EmitSeg2D {
   __device__ __forceinline__ void store(int pos, uint32_t, const Out& v) const { lg[pos] = v; }
 };
 
-// segB[e * (k2dLayers + 1) + L] = batch position of event e's first point of layer L, L in [0, 100]
-// (entry 100: the event's end). After the exclusive scan, hist[histBase + L * nbEvent] of the event's
-// first block is the position where digit L of the event begins; digit 100 has no points, so its
-// entry is where the event ends. Empty events have no blocks and get no entries.
-__global__ __launch_bounds__(kRadix) void k_layer_starts2d(const BlockDesc* __restrict__ blocks,
-                                                           const int* __restrict__ histScanned,
-                                                           int* __restrict__ segB) {
-  const BlockDesc bd = blocks[blockIdx.x];
-  const int L = (int)threadIdx.x;
-  if (bd.bLocal != 0 || L > k2dLayers) return;
-  segB[bd.event * (k2dLayers + 1) + L] = histScanned[bd.histBase + L * bd.nbEvent];
-}
-
 // Segment size classes of k_segsort2d: up to kSegSmall points (kSegThreadsS threads, 28 KB of LDS,
 // 5 workgroups per CU), up to kSegCap points (kSegThreadsL threads, 72 KB, 2 per CU), and larger
-// ones (kSegThreadsH threads, slots in global memory). Every class runs over all (event, layer)
-// segments and leaves the others at once.
+// ones (kSegThreadsH threads, slots in global memory). The two LDS classes run over all (event,
+// layer) segments and leave the others at once; the largest class, rare, takes its segments from a
+// list (k_layer_starts2d), so that its launch holds few empty workgroups.
 constexpr int kSegSmall = 2048;
 constexpr int kSegCap = 10240;
 constexpr int kSegThreadsS = 256;
 constexpr int kSegThreadsL = 1024;
 constexpr int kSegThreadsH = 512;
 constexpr int kSegNone = 0;  // "no upper bound": the class of the segments above kSegCap
+
+// segB[e * (k2dLayers + 1) + L] = batch position of event e's first point of layer L, L in [0, 100]
+// (entry 100: the event's end). After the exclusive scan, hist[histBase + L * nbEvent] of the event's
+// first block is the position where digit L of the event begins; digit 100 has no points, so its
+// entry is where the event ends. Empty events have no blocks and get no entries. Each layer of more
+// than kSegCap points is also listed, as e * k2dLayers + L, in segList, counted in *segCnt (zero
+// before the launch; one atomic per wave that has such layers).
+__global__ __launch_bounds__(kRadix) void k_layer_starts2d(const BlockDesc* __restrict__ blocks,
+                                                           const int* __restrict__ histScanned,
+                                                           int* __restrict__ segB, int* __restrict__ segList,
+                                                           int* __restrict__ segCnt) {
+  const BlockDesc bd = blocks[blockIdx.x];
+  if (bd.bLocal != 0) return;  // uniform over the block
+  const int L = (int)threadIdx.x;
+  const int* H = histScanned + bd.histBase;
+  const int start = L <= k2dLayers ? H[L * bd.nbEvent] : 0;
+  if (L <= k2dLayers) segB[bd.event * (k2dLayers + 1) + L] = start;
+  const bool big = L < k2dLayers && H[(L + 1) * bd.nbEvent] - start > kSegCap;
+  const uint64_t m = __ballot(big);  // every lane of the wave is here
+  if (m != 0ull) {
+    const int lane = lane_id(), leader = __builtin_ctzll(m);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(segCnt, __popcll(m));
+    base = __shfl(base, leader, kWave);
+    if (big) segList[base + __popcll(m & ((1ull << lane) - 1ull))] = bd.event * k2dLayers + L;
+  }
+}
 constexpr int kSegIlp = 4;   // points of a thread taken together in each loop over a segment
 // Points of a thread (positions tid + r * NT, r < kSegHold) whose records the kSegCap class keeps in
 // registers from the count pass to the output pass, so that they are read once; with the LDS of two
@@ -198,8 +213,9 @@ struct SegCounts {
   }
 };
 
-// Step (2) for the layer segment lg[start, end) of event e = blockIdx.x / 100, layer L = blockIdx.x %
-// 100, if its size n is in this class, (LO, CAP] (CAP = kSegNone: above LO). LayerTilesSerial::fill
+// Step (2) for the layer segment lg[start, end) of event e and layer L, id = e * 100 + L: id =
+// blockIdx.x, or the blockIdx.x-th entry of the class's list when one is given (workgroups past the
+// list's count leave at once), if its size n is in this class, (LO, CAP] (CAP = kSegNone: above LO). LayerTilesSerial::fill
 // (LayerTilesSerial.h:22) appends in index order, and the segment arrives in index order:
 //   counts of the 10^4 bins (LDS atomics); their exclusive scan is each bin's first position, written
 //   as the layer's bin-table entries; every point then takes the next free slot of its bin (LDS
@@ -216,7 +232,8 @@ template <int NT, int LO, int CAP>
 __global__ __launch_bounds__(NT) CLUE_WAVES_PER_EU(kSegHoldOf<CAP> > 0 ? 8 : 1) void k_segsort2d(const int* __restrict__ evOff, const int* __restrict__ segB,
                                                   const float4* __restrict__ lg, int* __restrict__ scr,
                                                   uint32_t* __restrict__ keyOut, float4* __restrict__ xyw,
-                                                  uint16_t* __restrict__ t16, int* __restrict__ t32, int64_t stride) {
+                                                  uint16_t* __restrict__ t16, int* __restrict__ t32, int64_t stride,
+                                                  const int* __restrict__ segList, const int* __restrict__ segCnt) {
   using C = SegCounts<NT, CAP>;
   constexpr bool kLds = CAP != kSegNone;
   constexpr int kPts = kLds ? CAP : 1;
@@ -224,7 +241,12 @@ __global__ __launch_bounds__(NT) CLUE_WAVES_PER_EU(kSegHoldOf<CAP> > 0 ? 8 : 1) 
   __shared__ uint16_t s_bin[kPts];
   __shared__ uint16_t s_slot[kPts];
   __shared__ int s_tmp[NT / kWave];
-  const int e = (int)blockIdx.x / k2dLayers, L = (int)blockIdx.x % k2dLayers;
+  int id = (int)blockIdx.x;
+  if (segList) {
+    if (id >= *segCnt) return;  // uniform over the block
+    id = segList[id];
+  }
+  const int e = id / k2dLayers, L = id % k2dLayers;
   const int evBegin = evOff[e], evEnd = evOff[e + 1];
   if (evBegin == evEnd) return;  // an empty event: nothing reads its table (uniform over the block)
   const int start = segB[e * (k2dLayers + 1) + L];

@@ -350,6 +350,7 @@ int launch_slot(clue_server* s, Slot& sl) {
     HIP_TRY(c, copy_in(s, sl, sl.reqs2, sl.offs2, kIn2, st + L.in2, sf2, din, B.f2));
     Clue2DBuffers b;
     clue2d_carve(dev + B.ws2, N, E2, (int)nb2, b);
+    clue2d_segment_bounds(b, sl.offs2.data(), E2);
     b.err = err2;
     b.errStride = 1;
     const std::vector<BlockDesc> blocks = make_blocks(sl.offs2.data(), E2);

@@ -67,19 +67,28 @@ int main(int argc, char** argv) {
       k_radix_scatter<LayerOf2D, DigitLayer2D, EmitSeg2D>(blocks.data(), layerOf, hist.data(),
                                                           EmitSeg2D{in.x.data(), in.y.data(), in.w.data(), lg.data()});
     });
-    emu::launch(nb, kRadix, [&] { k_layer_starts2d(blocks.data(), hist.data(), segB.data()); });
-    emu::launch(E * k2dLayers, kSegThreadsS, [&] {
+    const int nSeg = E * k2dLayers;
+    std::vector<int> segList((size_t)std::max(nSeg, 1), P), segCnt(1, 0);
+    emu::launch(nb, kRadix, [&] {
+      k_layer_starts2d(blocks.data(), hist.data(), segB.data(), segList.data(), segCnt.data());
+    });
+    // launch size of the largest class from the event sizes, as clue2d_segment_bounds
+    int gridH = 0;
+    for (int e = 0; e < E; ++e) gridH += (int)std::min<int64_t>(k2dLayers, (in.offs[e + 1] - in.offs[e]) / (kSegCap + 1));
+    emu::launch(nSeg, kSegThreadsS, [&] {
       k_segsort2d<kSegThreadsS, 0, kSegSmall>(evOff.data(), segB.data(), lg.data(), scr.data(), keyS.data(), xyw.data(),
-                                     table16.data(), table.data(), stride);
+                                              table16.data(), table.data(), stride, nullptr, nullptr);
     });
-    emu::launch(E * k2dLayers, kSegThreadsL, [&] {
-      k_segsort2d<kSegThreadsL, kSegSmall, kSegCap>(evOff.data(), segB.data(), lg.data(), scr.data(), keyS.data(), xyw.data(),
-                                           table16.data(), table.data(), stride);
+    emu::launch(nSeg, kSegThreadsL, [&] {
+      k_segsort2d<kSegThreadsL, kSegSmall, kSegCap>(evOff.data(), segB.data(), lg.data(), scr.data(), keyS.data(),
+                                                    xyw.data(), table16.data(), table.data(), stride, nullptr, nullptr);
     });
-    emu::launch(E * k2dLayers, kSegThreadsH, [&] {
-      k_segsort2d<kSegThreadsH, kSegCap, kSegNone>(evOff.data(), segB.data(), lg.data(), scr.data(), keyS.data(), xyw.data(),
-                                          table16.data(), table.data(), stride);
-    });
+    if (gridH > 0)
+      emu::launch(gridH, kSegThreadsH, [&] {
+        k_segsort2d<kSegThreadsH, kSegCap, kSegNone>(evOff.data(), segB.data(), lg.data(), scr.data(), keyS.data(),
+                                                     xyw.data(), table16.data(), table.data(), stride, segList.data(),
+                                                     segCnt.data());
+      });
     const int nTiles = nb * kTilesPerSort;
     std::vector<int4> reachD(nTiles, int4{P, P, P, P}), reachN(nTiles, int4{P, P, P, P});
     emu::launch((nTiles + kReachThreads - 1) / kReachThreads, kReachThreads, [&] {

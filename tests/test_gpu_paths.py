@@ -242,3 +242,16 @@ def test_index_pass_extremes(dev, oracle_lib, params):
     want = oracle_lib.clue2d_batch(packed, offs, dc, rhoc, odf)
     run_plan(dev, packed, offs, want, f"index pass extremes {params}", reps=2, p=clue_hip.params2d(dc, rhoc, odf),
              use_graph=True, index_pass=True)
+
+
+def test_many_layers_above_10240(dev, oracle_lib):
+    """Layers of the largest bin-sort class in both waves of an event's layer-start block (70 layers of
+    11,000 points), beside an event with three such layers and a small one: the class's segment list,
+    its count and its launch bound."""
+    evs = [layered_event([11_000] * 70, 21), layered_event([0, 20_000, 500, 15_000, 0, 10_241], 22),
+           synth.clue2d_event(3_000, 23)]
+    packed, offs = synth.pack(evs)
+    print(f"case clue2d many large layers n={int(offs[-1])}", flush=True)
+    want = oracle_lib.clue2d_batch(packed, offs)
+    run_plan(dev, packed, offs, want, "many large layers", reps=2, use_graph=True)
+    helpers.assert_bit_exact(run_batch(dev, packed, offs), want, helpers.OUT2D, "many large layers batch")
