@@ -397,7 +397,7 @@ def main():
     ap.add_argument("--per-event-reps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-per-event", action="store_true", help="skip the per-event comparison leg")
-    ap.add_argument("--serve-rates", default="500,1500", help="serving leg: event rates (events/s), comma-separated")
+    ap.add_argument("--serve-rates", default="500,1000,1500", help="serving leg: event rates (events/s), comma-separated")
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="serving leg: length of the stream")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-3d", action="store_true", help="skip the CLUE3D (512-event) leg")
