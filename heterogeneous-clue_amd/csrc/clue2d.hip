@@ -136,18 +136,18 @@ void labels_step(const Clue2DBuffers& b, const Clue2DIO& io, bool indexPass, hip
 
 void clue2d_enqueue(const Clue2DBuffers& b, const Clue2DIO& io, const Clue2DParams& prm, hipStream_t s) {
   if (b.N <= 0 || b.nBlocks <= 0) return;
-  clear_seed_words(b.seeds, b.N, s);  // the nearest-higher kernel sets one bit per seed
   const int nHist = kRadix * b.nBlocks;
-  // tile order: a stable pass by layer into {x, y, w, index} records, then each (event, layer) sorted
-  // by bin in its own workgroup, which also writes that layer's bin-table entries
+  // tile order: a stable pass by layer into {x, y, w, index} records (its histogram kernel also
+  // clears the seed words the nearest-higher kernel sets and the largest class's list count; its
+  // scatter writes each event's layer starts and that list), then each (event, layer) sorted by bin
+  // in its own workgroup, which also writes that layer's bin-table entries
   const LayerOf2D layerOf{io.layer, b.err, b.errStride};
-  k_radix_hist<LayerOf2D, DigitLayer2D><<<b.nBlocks, kThreads, 0, s>>>(b.blocks, layerOf, b.hist);
+  const Prologue2D pro{b.seeds.word, (int64_t)seed_tiles(b.N) * 32, b.N, b.segCnt};
+  k_radix_hist<LayerOf2D, DigitLayer2D, Prologue2D><<<b.nBlocks, kThreads, 0, s>>>(b.blocks, layerOf, b.hist, pro);
   scan_exclusive(b.hist, b.hist, nHist, b.part, s);
   k_radix_scatter<LayerOf2D, DigitLayer2D, EmitSeg2D><<<b.nBlocks, kThreads, 0, s>>>(
-      b.blocks, layerOf, b.hist, EmitSeg2D{io.x, io.y, io.weight, b.lg});
+      b.blocks, layerOf, b.hist, EmitSeg2D{io.x, io.y, io.weight, b.lg, b.segB, b.segList, b.segCnt});
   const int nSeg = b.E * k2dLayers;
-  fill_i32(b.segCnt, 0, 1, s);
-  k_layer_starts2d<<<b.nBlocks, kRadix, 0, s>>>(b.blocks, b.hist, b.segB, b.segList, b.segCnt);
   k_segsort2d<kSegThreadsS, 0, kSegSmall><<<nSeg, kSegThreadsS, 0, s>>>(
       b.evOff, b.segB, b.lg, b.scr, b.keyS, b.xyw, b.table16, b.table, b.tableStride, nullptr, nullptr);
   k_segsort2d<kSegThreadsL, kSegSmall, kSegCap><<<nSeg, kSegThreadsL, 0, s>>>(

@@ -28,7 +28,8 @@ namespace g2d {
 // in two steps:
 //  (1) a stable counting-sort pass by layer (the radix-pass kernels of tiles.h with the layer as the
 //      digit), whose output is each point as {x, y, w, index bits}, grouped by (event, layer) and in
-//      index order inside a layer (k_layer_starts2d then lists where each layer begins);
+//      index order inside a layer (the first block of each event also writes where each layer
+//      begins);
 //  (2) k_segsort2d: one workgroup per (event, layer) orders that layer's points by bin, keeping index
 //      order inside a bin, writes the sorted {x, y, w, index} records (and the bin keys of each
 //      256-point tile's first and last position), and writes the layer's 10^4 entries of the
@@ -59,8 +60,16 @@ struct EmitSeg2D {
   const float* y;
   const float* w;
   float4* lg;
+  int* segB;     // per event: the start of each layer and the event's end (k2dLayers + 1 entries)
+  int* segList;  // the layers above kSegCap points, as e * k2dLayers + L
+  int* segCnt;   // their count (zero before the pass)
   using Payload = int;
   using Out = float4;
+  // The first block of each event (bLocal 0) holds the event's digit bases: base[L] is where layer L
+  // of the event begins and base[k2dLayers] (a digit no point has) where the event ends. Such a
+  // block writes them as segB and lists the layers above kSegCap points (one atomic per wave that
+  // has any); called by every thread of the block.
+  __device__ __forceinline__ void block_starts(const BlockDesc& bd, const int* base) const;
   __device__ __forceinline__ Payload load(int i) const { return i; }
   __device__ __forceinline__ Out fetch(int idx) const { return make_float4(x[idx], y[idx], w[idx], __int_as_float(idx)); }
   __device__ __forceinline__ void store(int pos, uint32_t, const Out& v) const { lg[pos] = v; }
@@ -70,7 +79,8 @@ struct EmitSeg2D {
 // 5 workgroups per CU), up to kSegCap points (kSegThreadsL threads, 72 KB, 2 per CU), and larger
 // ones (kSegThreadsH threads, slots in global memory). The two LDS classes run over all (event,
 // layer) segments and leave the others at once; the largest class, rare, takes its segments from a
-// list (k_layer_starts2d), so that its launch holds few empty workgroups.
+// list (written by the layer pass, EmitSeg2D::block_starts), so that its launch holds few empty
+// workgroups.
 constexpr int kSegSmall = 2048;
 constexpr int kSegCap = 10240;
 constexpr int kSegThreadsS = 256;
@@ -78,32 +88,38 @@ constexpr int kSegThreadsL = 1024;
 constexpr int kSegThreadsH = 512;
 constexpr int kSegNone = 0;  // "no upper bound": the class of the segments above kSegCap
 
-// segB[e * (k2dLayers + 1) + L] = batch position of event e's first point of layer L, L in [0, 100]
-// (entry 100: the event's end). After the exclusive scan, hist[histBase + L * nbEvent] of the event's
-// first block is the position where digit L of the event begins; digit 100 has no points, so its
-// entry is where the event ends. Empty events have no blocks and get no entries. Each layer of more
-// than kSegCap points is also listed, as e * k2dLayers + L, in segList, counted in *segCnt (zero
-// before the launch; one atomic per wave that has such layers).
-__global__ __launch_bounds__(kRadix) void k_layer_starts2d(const BlockDesc* __restrict__ blocks,
-                                                           const int* __restrict__ histScanned,
-                                                           int* __restrict__ segB, int* __restrict__ segList,
-                                                           int* __restrict__ segCnt) {
-  const BlockDesc bd = blocks[blockIdx.x];
+__device__ __forceinline__ void EmitSeg2D::block_starts(const BlockDesc& bd, const int* base) const {
   if (bd.bLocal != 0) return;  // uniform over the block
   const int L = (int)threadIdx.x;
-  const int* H = histScanned + bd.histBase;
-  const int start = L <= k2dLayers ? H[L * bd.nbEvent] : 0;
-  if (L <= k2dLayers) segB[bd.event * (k2dLayers + 1) + L] = start;
-  const bool big = L < k2dLayers && H[(L + 1) * bd.nbEvent] - start > kSegCap;
+  if (L <= k2dLayers) segB[bd.event * (k2dLayers + 1) + L] = base[L];
+  const bool big = L < k2dLayers && base[L + 1] - base[L] > kSegCap;
   const uint64_t m = __ballot(big);  // every lane of the wave is here
   if (m != 0ull) {
     const int lane = lane_id(), leader = __builtin_ctzll(m);
-    int base = 0;
-    if (lane == leader) base = atomicAdd(segCnt, __popcll(m));
-    base = __shfl(base, leader, kWave);
-    if (big) segList[base + __popcll(m & ((1ull << lane) - 1ull))] = bd.event * k2dLayers + L;
+    int first = 0;
+    if (lane == leader) first = atomicAdd(segCnt, __popcll(m));
+    first = __shfl(first, leader, kWave);
+    if (big) segList[first + __popcll(m & ((1ull << lane) - 1ull))] = bd.event * k2dLayers + L;
   }
 }
+
+// Once per block of the layer pass's histogram kernel: the block's words of the seed bits (which the
+// nearest-higher kernel sets with atomics; a word shared with the next block is cleared by both), the
+// padding words after the batch's last point (the block that ends the batch), and the count of the
+// largest bin-sort class's list (the block that starts the batch).
+struct Prologue2D {
+  uint64_t* word;
+  int64_t nWords;
+  int64_t N;
+  int* segCnt;
+  __device__ __forceinline__ void operator()(const BlockDesc& bd) const {
+    const int64_t w0 = (int64_t)bd.begin >> 6;
+    const int64_t w1 = bd.end == N ? nWords : ((int64_t)bd.end + 63) >> 6;
+    for (int64_t k = w0 + (int64_t)threadIdx.x; k < w1; k += kThreads) word[k] = 0ull;
+    if (bd.begin == 0 && threadIdx.x == 0) *segCnt = 0;
+  }
+};
+
 constexpr int kSegIlp = 4;   // points of a thread taken together in each loop over a segment
 // Points of a thread (positions tid + r * NT, r < kSegHold) whose records the kSegCap class keeps in
 // registers from the count pass to the output pass, so that they are read once; with the LDS of two

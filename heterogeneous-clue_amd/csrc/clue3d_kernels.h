@@ -67,6 +67,7 @@ struct EmitLast3D {
     int idx;
     float4 v;
   };
+  __device__ __forceinline__ void block_starts(const BlockDesc&, const int*) const {}
   __device__ __forceinline__ Payload load(int i) const { return iIn[i]; }
   __device__ __forceinline__ Out fetch(int idx) const {
     return Out{idx, make_float4(roz[idx], phi[idx], en[idx], z[idx])};

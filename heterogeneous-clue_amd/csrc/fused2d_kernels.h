@@ -45,6 +45,7 @@ struct EmitPoint2D {
   float4* lg;
   using Payload = float4;
   using Out = float4;
+  __device__ __forceinline__ void block_starts(const BlockDesc&, const int*) const {}
   __device__ __forceinline__ Payload load(int i) const { return make_float4(x[i], y[i], w[i], __int_as_float(i)); }
   __device__ __forceinline__ Out fetch(const Payload& v) const { return v; }
   __device__ __forceinline__ void store(int pos, uint32_t, const Out& v) const { lg[pos] = v; }

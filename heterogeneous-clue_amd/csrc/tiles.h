@@ -61,11 +61,18 @@ struct KeyArray {
   __device__ __forceinline__ uint32_t operator()(int i, int) const { return key[i]; }
 };
 
-template <class Src, class D>
+// Work a histogram pass does once per block beside its counts (clearing what later kernels of the
+// same launch sequence accumulate into), so that it needs no launch of its own.
+struct NoPrologue {
+  __device__ __forceinline__ void operator()(const BlockDesc&) const {}
+};
+
+template <class Src, class D, class Pro = NoPrologue>
 __global__ __launch_bounds__(kThreads) void k_radix_hist(const BlockDesc* __restrict__ blocks, Src src,
-                                                         int* __restrict__ hist) {
+                                                         int* __restrict__ hist, Pro pro = Pro{}) {
   __shared__ int s_h[kRadix];
   const BlockDesc bd = blocks[xcd_block((int)blockIdx.x, (int)gridDim.x)];
+  pro(bd);
   if (threadIdx.x < kRadix) s_h[threadIdx.x] = 0;
   // all of the thread's keys in flight at once
   int d[kSortItems];
@@ -126,6 +133,7 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const BlockDesc* __r
   }
   if (tid < kRadix) s_base[tid] = histScanned[bd.histBase + tid * bd.nbEvent + bd.bLocal];
   for (int k = lane; k < kRadix; k += kWave) s_cnt[wave][k] = 0;
+  // (s_base is read by emit.block_starts after the ranking's barrier)
   // Stable rank inside the wave: per round, the lanes of one digit (ballot match) take consecutive
   // ranks after that digit's running count in the wave; the digit's lowest lane advances the count
   // and broadcasts the old value (the shuffle orders every update of a round before the next
@@ -146,6 +154,8 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const BlockDesc* __r
     rank[r] = old + before;
   }
   __syncthreads();
+  // the block's digit bases are in LDS now: an Emit may use them (uniform over the block)
+  emit.block_starts(bd, s_base);
   // the waves' counts of each digit, in wave order: offsets and the block total
   if (tid < kRadix) {
     int run = 0;
@@ -201,6 +211,7 @@ struct EmitFirst {
   int* iOut;
   using Payload = int;
   using Out = int;
+  __device__ __forceinline__ void block_starts(const BlockDesc&, const int*) const {}
   __device__ __forceinline__ Payload load(int i) const { return i; }
   __device__ __forceinline__ Out fetch(Payload v) const { return v; }
   __device__ __forceinline__ void store(int pos, uint32_t k, Out v) const {
@@ -216,6 +227,7 @@ struct EmitMid {
   int* iOut;
   using Payload = int;
   using Out = int;
+  __device__ __forceinline__ void block_starts(const BlockDesc&, const int*) const {}
   __device__ __forceinline__ Payload load(int i) const { return iIn[i]; }
   __device__ __forceinline__ Out fetch(Payload v) const { return v; }
   __device__ __forceinline__ void store(int pos, uint32_t k, Out v) const {
@@ -282,15 +294,13 @@ __device__ __forceinline__ int rank_of(const SeedRank& sr, int i) {
 
 // clusterIndex[i] = isSeed[root(i)] ? (#seeds of the event before root) : -1, where root(i) is
 // reached by following parent[] (parent[i] == i at roots). `part` needs scan_partials(seed_tiles(N))
-// ints. isSeed == nullptr: the seed words are already set (cleared by clear_seed_words, then one bit
-// per seed set by the caller's kernels).
+// ints. isSeed == nullptr: the seed words are already set (cleared, then one bit per seed set by the
+// caller's kernels).
 void launch_labels(const BlockDesc* blocksDev, int nBlocks, const int* parent, const int* isSeed, int64_t N,
                    const int* evOffDev, int32_t* label, const SeedRank& sr, int* part, hipStream_t s);
 // The seed words' prefixes and scanned tile counts alone (the words already set), for callers with
 // their own labels kernel.
 void launch_seed_ranks(int64_t N, const SeedRank& sr, int* part, hipStream_t s);
-// Zero the seed words of N points on the stream (a fill kernel, no memset node in a captured graph).
-void clear_seed_words(const SeedRank& sr, int64_t N, hipStream_t s);
 
 // Fill an int buffer on the stream with a kernel (graph-capture friendly, no memset nodes).
 void fill_i32(int* p, int value, int64_t n, hipStream_t s);

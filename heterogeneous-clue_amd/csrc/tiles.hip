@@ -70,11 +70,6 @@ void launch_seed_ranks(int64_t N, const SeedRank& sr, int* part, hipStream_t s) 
   seed_ranks(nullptr, N, sr, part, s);
 }
 
-void clear_seed_words(const SeedRank& sr, int64_t N, hipStream_t s) {
-  if (N <= 0) return;
-  fill_i32(reinterpret_cast<int*>(sr.word), 0, (int64_t)seed_tiles(N) * 32 * 2, s);
-}
-
 void fill_i32(int* p, int value, int64_t n, hipStream_t s) {
   if (n <= 0) return;
   k_fill<<<(unsigned)((n + kThreads - 1) / kThreads), kThreads, 0, s>>>(p, value, n);

@@ -61,16 +61,18 @@ int main(int argc, char** argv) {
   if (nb > 0) {
     const int nHist = kRadix * nb;
     const LayerOf2D layerOf{in.layer.data(), &err, 0};
-    emu::launch(nb, kThreads, [&] { k_radix_hist<LayerOf2D, DigitLayer2D>(blocks.data(), layerOf, hist.data()); });
+    const int nSeg = E * k2dLayers;
+    std::vector<int> segList((size_t)std::max(nSeg, 1), P), segCnt(1, P);
+    SeedRankHost seeds(N);  // poisoned: the histogram kernel's prologue clears the words
+    const Prologue2D pro{seeds.word.data(), (int64_t)seed_tiles(N) * 32, N, segCnt.data()};
+    emu::launch(nb, kThreads, [&] {
+      k_radix_hist<LayerOf2D, DigitLayer2D, Prologue2D>(blocks.data(), layerOf, hist.data(), pro);
+    });
     scan_exclusive(hist.data(), hist.data(), nHist, part.data());
     emu::launch(nb, kThreads, [&] {
-      k_radix_scatter<LayerOf2D, DigitLayer2D, EmitSeg2D>(blocks.data(), layerOf, hist.data(),
-                                                          EmitSeg2D{in.x.data(), in.y.data(), in.w.data(), lg.data()});
-    });
-    const int nSeg = E * k2dLayers;
-    std::vector<int> segList((size_t)std::max(nSeg, 1), P), segCnt(1, 0);
-    emu::launch(nb, kRadix, [&] {
-      k_layer_starts2d(blocks.data(), hist.data(), segB.data(), segList.data(), segCnt.data());
+      k_radix_scatter<LayerOf2D, DigitLayer2D, EmitSeg2D>(
+          blocks.data(), layerOf, hist.data(),
+          EmitSeg2D{in.x.data(), in.y.data(), in.w.data(), lg.data(), segB.data(), segList.data(), segCnt.data()});
     });
     // launch size of the largest class from the event sizes, as clue2d_segment_bounds
     int gridH = 0;
@@ -110,8 +112,6 @@ int main(int argc, char** argv) {
     // the links and records reuse the layer-order records and the bin sort's slots, as clue2d.hip does
     int4* rec = reinterpret_cast<int4*>(lg.data());
     int* parent = indexPass ? scr.data() : reinterpret_cast<int*>(lg.data());
-    SeedRankHost seeds(N);
-    std::fill(seeds.word.begin(), seeds.word.end(), 0ull);  // clear_seed_words
     dispatch(nv, [&](auto T, auto I, auto S) {
       emu::launch(CompShape<T, I>::kPerSort * nb, T, [&] {
         if (indexPass)
