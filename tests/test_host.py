@@ -84,3 +84,17 @@ def test_library_reports_no_device_without_gpu():
         assert lib.clue_create(0, 0) is None
         p = lib.clue3d_default_params()
         assert p.densitySiblingLayers == 3 and abs(p.densityXYDistanceSqr - 3.24) < 1e-6
+
+
+def test_header_constants_match_the_binding():
+    """Every #define of include/clue_hip.h has the value the Python binding uses."""
+    import clue_hip
+    hdr = (REPO / "include" / "clue_hip.h").read_text()
+    defs = {k: int(v) for k, v in re.findall(r"#define (CLUE_\w+) \(?(-?\d+)\)?", hdr)}
+    py = {"CLUE_OK": clue_hip.CLUE_OK, "CLUE_FUSED_CAPACITY": clue_hip.FUSED_CAPACITY,
+          "CLUE_NOALLOC_MAX_POINTS": clue_hip.NOALLOC_MAX_POINTS}
+    py.update({name: code for code, name in clue_hip.ERRORS.items()})
+    py.update({f"CLUE_{k}": getattr(clue_hip, k) for k in dir(clue_hip) if k.startswith(("PATH_", "PLAN_"))})
+    assert sorted(defs) == sorted(py), (sorted(defs), sorted(py))
+    for k, v in defs.items():
+        assert py[k] == v, k
