@@ -240,22 +240,24 @@ __device__ __forceinline__ int list_layer(const int* __restrict__ T, const Win3D
 }
 
 // Walks the cnt (> 0 or none) listed runs, calling f.run(layerOff) at the start of each run and
-// f(q0, q1, two) for the candidates q0 and (if two) q1 = q0 + 1. The next entry is read ahead.
+// f.pair(xo, two) for the staged candidates at {r_over_absz, phi} byte offset xo (8 bytes per
+// candidate) and, if two, the next one: a step costs one compare and two adds of bookkeeping. The
+// second candidate of a run's last step is read but masked off; the staged range is kept below
+// the staging capacity, so that read stays inside the arrays. The next entry is read ahead.
 template <class F>
 __device__ __forceinline__ void walk_list(const uint32_t* list, int cnt, F& f) {
   if (cnt <= 0) return;
   uint32_t e = list[0];
-  int q = (int)(e & kOffMask), t = (int)((e >> kOffBits) & kOffMask);
+  int xo = (int)(e & kOffMask) << 3, tx = (int)((e >> kOffBits) & kOffMask) << 3;
   f.run((int)(e >> (2 * kOffBits)));
   int j = 1;
   e = list[(cnt > 1 ? 1 : 0) * kThreads];
-  while (q < t) {
-    const bool two = q + 1 < t;
-    f(q, two ? q + 1 : q, two);
-    q += two ? 2 : 1;
-    if (q >= t && j < cnt) {
-      q = (int)(e & kOffMask);
-      t = (int)((e >> kOffBits) & kOffMask);
+  while (xo < tx) {
+    f.pair(xo, xo + 8 < tx);
+    xo += 16;
+    if (xo >= tx && j < cnt) {
+      xo = (int)(e & kOffMask) << 3;
+      tx = (int)((e >> kOffBits) & kOffMask) << 3;
       f.run((int)(e >> (2 * kOffBits)));
       ++j;
       e = list[(j < cnt ? j : cnt - 1) * kThreads];
@@ -314,6 +316,15 @@ struct Staged3 {
     const float2 a = xy[q];
     return Cand3{a.x, a.y, z[q]};
   }
+  // the candidates at xy byte offset xo (z at xo / 2) and the one after it
+  __device__ __forceinline__ void pair(int xo, Cand3& c0, Cand3& c1) const {
+    const char* bx = reinterpret_cast<const char*>(xy) + xo;
+    const char* bz = reinterpret_cast<const char*>(z) + (xo >> 1);
+    const float2 a = *reinterpret_cast<const float2*>(bx);
+    const float2 b = *reinterpret_cast<const float2*>(bx + 8);
+    c0 = Cand3{a.x, a.y, *reinterpret_cast<const float*>(bz)};
+    c1 = Cand3{b.x, b.y, *reinterpret_cast<const float*>(bz + 4)};
+  }
 };
 // Candidates read from the tile-order records in global memory (unstaged blocks).
 struct Global4 {
@@ -359,25 +370,27 @@ struct DensityStep3D {
   float rho;
   __device__ __forceinline__ void run(int layerOff) { kdfRun = (minL + layerOff == L) ? kdfSame : kdf; }
   template <class C>
-  __device__ __forceinline__ float term(const C& c, float dp, int q) const {
+  __device__ __forceinline__ float term(const C& c, float dp, bool self) const {
     const float r1 = c.x * zi;
     const float d = (r0 - r1) * (r0 - r1) + r1 * r1 * dp * dp;
-    const float e = ((q == p) ? 1.f : kdfRun) * c.z;
+    const float e = (self ? 1.f : kdfRun) * c.z;
     return d < dsq ? e : 0.f;
   }
-  __device__ __forceinline__ void operator()(int q0, int q1, bool two) {
-    const auto c0 = src[q0];
-    const auto c1 = src[q1];
+  // staged candidates at byte offset xo and (if two) xo + 8 (see walk_list)
+  __device__ __forceinline__ void pair(int xo, bool two) {
+    Cand3 c0, c1;
+    src.pair(xo, c0, c1);
     float dp0 = phii - c0.y, dp1 = phii - c1.y;
     reduce_pair(dp0, dp1);
-    const float e0 = term(c0, dp0, q0);
-    const float e1 = term(c1, dp1, q1);
+    const int self = p << 3;
+    const float e0 = term(c0, dp0, xo == self);
+    const float e1 = term(c1, dp1, xo + 8 == self);
     rho += e0;
     rho += two ? e1 : 0.f;
   }
   __device__ __forceinline__ void one(int q) {
     const auto c = src[q];
-    rho += term(c, reduce_range(phii - c.y), q);
+    rho += term(c, reduce_range(phii - c.y), q == p);
   }
 };
 
@@ -423,7 +436,7 @@ __global__ __launch_bounds__(kThreads) void k_density3d(
   }
   int rlo, rhi;
   block_range(lo, hi, s_red, rlo, rhi);
-  const bool staged = S <= kMaxListS && rhi - rlo <= STAGE;
+  const bool staged = S <= kMaxListS && rhi - rlo < STAGE;  // a spare slot for the walk's masked read
   if (staged && rhi > rlo) stage_range3<STAGE>(cre, s_xy, s_z, rlo, rhi);
   __syncthreads();
   const float kdf = prm.kernelDensityFactor;
@@ -474,13 +487,14 @@ struct NhStep3D {
     bq = upd ? q : bq;
     bo = upd ? cur : bo;
   }
-  __device__ __forceinline__ void operator()(int q0, int q1, bool two) {
-    const auto c0 = src[q0];
-    const auto c1 = src[q1];
+  // staged candidates at byte offset xo and (if two) xo + 8 (see walk_list); bq keeps the byte offset
+  __device__ __forceinline__ void pair(int xo, bool two) {
+    Cand3 c0, c1;
+    src.pair(xo, c0, c1);
     float dp0 = phii - c0.y, dp1 = phii - c1.y;
     reduce_pair(dp0, dp1);
-    take(c0, dp0, q0, true);
-    take(c1, dp1, q1, two);
+    take(c0, dp0, xo, true);
+    take(c1, dp1, xo + 8, two);
   }
   __device__ __forceinline__ void one(int q) {
     const auto c = src[q];
@@ -534,7 +548,7 @@ __global__ __launch_bounds__(kThreads) void k_nh3d(
   }
   int rlo, rhi;
   block_range(lo, hi, s_red, rlo, rhi);
-  const bool staged = S <= kMaxListS && rhi - rlo <= STAGE;
+  const bool staged = S <= kMaxListS && rhi - rlo < STAGE;  // a spare slot for the walk's masked read
   if (staged && rhi > rlo) stage_range3<STAGE>(crp, s_xy, s_z, rlo, rhi);
   __syncthreads();
 #pragma unroll
@@ -552,7 +566,7 @@ __global__ __launch_bounds__(kThreads) void k_nh3d(
         NhStep3D<Staged3> f{me[r].x * zi, zi, me[r].y, rhoi, Staged3{s_xy, s_z}, FLT_MAX, -1, 0, 0};
         walk_window<1>(T, win[r], skipL, rlo, s_runs + threadIdx.x, f);
         best = f.best;
-        bq = f.bq + rlo;
+        bq = (f.bq >> 3) + rlo;
         bo = f.bo;
       } else {
         NhStep3D<Global4> f{me[r].x * zi, zi, me[r].y, rhoi, Global4{crp}, FLT_MAX, -1, 0, 0};
